@@ -1,0 +1,205 @@
+"""Closed-form checks, invariants and analytic solutions that pin the oracle
+beyond the golden vectors.
+
+Each check is independent of the formula it tests: numpy's zero-padded diff,
+scipy's matrix log/exp of rotations, decimal matrices printed in the
+reference's test, and the textbook axial-stretching / Timoshenko solutions.
+"""
+import numpy as np
+import pytest
+from scipy.spatial.transform import Rotation
+
+from oracle import pyoracle
+
+
+# --- calculus kernels (Test_CalculusKernels.cpp:28-201) ----------------------
+
+@pytest.mark.parametrize("n", [2, 3, 7, 16, 135])
+def test_two_point_difference_zero_padded(n):
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((3, n))
+    got = pyoracle.two_point_difference(a)
+    want = np.diff(np.pad(a, ((0, 0), (1, 1))), axis=1)  # f(a) = f(b) = 0 ends
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("n", [2, 5, 64, 135])
+def test_quadrature_constant_field_halves_ends(n):
+    got = pyoracle.quadrature(np.full((3, n), 3.0))
+    assert np.all(got[:, 0] == 1.5) and np.all(got[:, -1] == 1.5)
+    assert np.all(got[:, 1:-1] == 3.0)
+
+
+def test_quadrature_is_trapezoid_of_zero_padded():
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((3, 40))
+    p = np.pad(a, ((0, 0), (1, 1)))
+    assert np.array_equal(pyoracle.quadrature(a), 0.5 * (p[:, 1:] + p[:, :-1]))
+
+
+# --- InvRotateDivide (Test_InvRotateDivide.cpp:35-83) -------------------------
+
+def _rotate_via_axis_angle(axis, angle):
+    """R = I + s K + (1-c) K^2, s = -sin(angle |axis|), K = [axis/|axis|]x
+    (elastica/Utilities/Geometry/AxisAngleRotation.cpp)."""
+    axis = np.asarray(axis, float)
+    n = axis / np.linalg.norm(axis)
+    th = angle * np.linalg.norm(axis)
+    K = np.array([[0, -n[2], n[1]], [n[2], 0, -n[0]], [-n[1], n[0], 0]])
+    return np.eye(3) + (-np.sin(th)) * K + (1 - np.cos(th)) * (K @ K)
+
+
+def test_inv_rotate_divide_axis_angle_closed_form():
+    n = 32
+    Q = np.stack([_rotate_via_axis_angle([1.0, 0.0, 1.0], k * np.pi / 20) for k in range(n)], axis=2)
+    kappa = pyoracle.inv_rotate_divide(Q, np.full(n - 1, 0.25))
+    # consecutive directors differ by sqrt(2) pi/20 about (1,0,1)/sqrt(2): |kappa| D0 = that angle
+    want = np.pi / 5 * np.array([1.0, 0.0, 1.0])
+    assert np.allclose(kappa, want[:, None], rtol=1e-7, atol=1e-12)
+
+
+def test_inv_rotate_divide_is_log_of_relative_rotation():
+    """kappa_k = vee(log(Q_k Q_{k+1}^T)) / D0_k on random directors."""
+    rng = np.random.default_rng(11)
+    n = 50
+    R = Rotation.random(n, random_state=12).as_matrix()
+    # small relative rotations as along a rod
+    Qs = [R[0]]
+    for k in range(1, n):
+        step = Rotation.from_rotvec(rng.normal(0, 0.3, 3)).as_matrix()
+        Qs.append(step @ Qs[-1])
+    Q = np.stack(Qs, axis=2)
+    D0 = rng.uniform(0.01, 0.2, n - 1)
+    kappa = pyoracle.inv_rotate_divide(Q, D0)
+    want = np.stack([Rotation.from_matrix(Q[:, :, k] @ Q[:, :, k + 1].T).as_rotvec() / D0[k]
+                     for k in range(n - 1)], axis=1)
+    assert np.allclose(kappa, want, rtol=1e-8, atol=1e-8)
+
+
+# --- Rodrigues director update (Test_SO3PrimitiveAddAssign.cpp:36-134) -------
+
+EXPECTED_1 = np.array([[0.25450619348149084, 0.9153737692840798, -0.31195746502722094],
+                       [-0.8348342628404866, 0.3707851059094285, 0.40690313199004513],
+                       [0.488137635372581, 0.15687341311510705, 0.8585524918084395]])
+EXPECTED_2 = np.array([[-0.8516901626985716, 0.5234374213384307, 0.025241487773656174],
+                       [-0.32338986831225147, -0.5628715080170252, 0.7606540991381047],
+                       [0.41236253447111093, 0.6396787720451501, 0.6486664849977726]])
+
+
+def test_so3_add_assign_decimal_kat():
+    n = 21
+    u = np.array([1.0, 3.2, 7.0])
+    u = u / np.linalg.norm(u) * (76.0 * np.pi / 180.0)
+    U = np.repeat(u[:, None], n, axis=1)
+    Q = np.repeat(np.eye(3)[:, :, None], n, axis=2)
+    Q = pyoracle.so3_add_assign(Q, U)
+    # the framework's default approx: epsilon 100 DBL_EPSILON, scale 1
+    # (tests/Unit/Framework/TestingFramework.hpp:76-79): |a-b| <= eps (1 + |b|)
+    eps = 100 * np.finfo(float).eps
+    assert np.all(np.abs(Q - EXPECTED_1[:, :, None]) <= eps * (1.0 + np.abs(EXPECTED_1[:, :, None])))
+    Q = pyoracle.so3_add_assign(Q, U)
+    assert np.all(np.abs(Q - EXPECTED_2[:, :, None]) <= eps * (1.0 + np.abs(EXPECTED_2[:, :, None])))
+
+
+def test_so3_add_assign_is_exp_of_minus_u():
+    rng = np.random.default_rng(5)
+    n = 64
+    Q0 = Rotation.random(n, random_state=6).as_matrix().transpose(1, 2, 0).copy()
+    U = rng.normal(0, 0.7, (3, n))
+    Q = pyoracle.so3_add_assign(Q0.copy(), U)
+    for k in range(n):
+        want = Rotation.from_rotvec(-U[:, k]).as_matrix() @ Q0[:, :, k]
+        assert np.allclose(Q[:, :, k], want, rtol=0, atol=5e-14)
+
+
+# --- invariants on rods ----------------------------------------------------------
+
+def _random_rods(sysm, rng):
+    for r, n in enumerate(sysm.n_elems):
+        sysm.init_straight_rod(r, 0.02, 0.5 + 0.1 * r, [0.1 * r, 0, 0], [1, 0, 0], [0, 0, 1],
+                               1000.0, 1e6, 1e6 / 1.5, 0.3, 0.3)
+        sysm.rod_field(r, "ReferenceCurvature")[...] = rng.normal(0, 0.5, (3, n - 1))
+        sysm.rod_field(r, "Velocity")[...] = rng.normal(0, 1e-2, (3, n + 1))
+        sysm.rod_field(r, "AngularVelocity")[...] = rng.normal(0, 1e-1, (3, n))
+
+
+STATE = ("Position", "Velocity", "Director", "AngularVelocity")
+
+
+def test_rod_in_block_equals_rod_alone_bitwise():
+    n_elems = [7, 12, 5, 30]
+    a = pyoracle.System(n_elems, max_block_elems=1024)
+    b = pyoracle.System(n_elems, max_block_elems=0)
+    assert a.n_blocks == 1 and b.n_blocks == len(n_elems)
+    _random_rods(a, np.random.default_rng(1))
+    _random_rods(b, np.random.default_rng(1))
+    a.step(1e-5, 50)
+    b.step(1e-5, 50)
+    for r in range(len(n_elems)):
+        for f in STATE + ("InternalLoads", "InternalTorques", "Curvature"):
+            assert np.array_equal(a.rod_field(r, f), b.rod_field(r, f)), (r, f)
+
+
+def test_undamped_internal_forces_telescope_to_zero():
+    s = pyoracle.System([20], damped=False)
+    _random_rods(s, np.random.default_rng(2))
+    s.step(1e-5, 3)
+    s.compute_internal_forces()
+    f = s.rod_field(0, "InternalLoads")
+    assert np.all(np.abs(f.sum(axis=1)) <= 1e-12 * np.abs(f).max())
+
+
+def test_straight_unstressed_rod_has_zero_loads():
+    s = pyoracle.System([16])
+    s.init_straight_rod(0, 0.05, 1.0, [0, 0, 0], [0, 0, 1], [0, 1, 0], 1e3, 1e6, 4e5, 0.1, 0.1)
+    s.step(1e-4, 10)
+    for f in ("InternalLoads", "InternalTorques", "Velocity", "AngularVelocity"):
+        assert not s.rod_field(0, f).any(), f
+
+
+# --- analytic solutions ---------------------------------------------------------
+
+def test_axial_stretching_tip_displacement():
+    """Clamped rod pulled at the tip: steady elongation F L / (E A - F)
+    (examples/axial_stretching/convergence.py:62-74)."""
+    n, L, r, E, F = 10, 1.0, 0.025, 1e4, 1.0
+    s = pyoracle.System([n])
+    s.init_straight_rod(0, r, L, [0, 0, 0], [1, 0, 0], [0, 1, 0], 1e3, E, E / 1.5, 10.0, 10.0)
+    s.add_clamp(0, 0, s.rod_field(0, "Position")[:, 0].copy(), s.rod_field(0, "Director")[:, 0].copy())
+    s.add_end_force(0, [F, 0, 0], ramp=1, start=0.0, interval=2.0)
+    dt = 0.01 * L / n
+    s.step(dt, int(20.0 / dt))
+    A = np.pi * r * r
+    want = F * L / (A * E - F)
+    tip = s.rod_field(0, "Position")[0, -1] - L
+    assert abs(tip - want) <= 1e-6 * want
+
+
+def _timoshenko_error(n):
+    L, r, rho, E, G, F = 3.0, 0.25, 5e3, 1e6, 1e4, -15.0
+    s = pyoracle.System([n])
+    s.init_straight_rod(0, r, L, [0, 0, 0], [0, 0, 1], [0, 1, 0], rho, E, G, 0.5, 0.5)
+    s.add_clamp(0, 0, s.rod_field(0, "Position")[:, 0].copy(), s.rod_field(0, "Director")[:, 0].copy())
+    s.add_end_force(0, [F, 0, 0], ramp=2, start=0.0, interval=100.0)
+    dt = 0.01 * L / n
+    s.step(dt, int(200.0 / dt))
+    x = s.rod_field(0, "Position")
+    arg = x[2]
+    A = np.pi * r * r
+    I = A * A / (4 * np.pi)
+    f = abs(F)
+    # the tip load points along -x, so the rod deflects along -x
+    want = arg * (-f / ((13.5 / 14.0) * G * A) + arg * (-f * L / 2 / (E * I) + arg * f / 6 / (E * I)))
+    return np.abs(x[0] - want).max(), np.abs(want).max()
+
+
+def test_timoshenko_cantilever_deflection_converges():
+    """Clamped shearable cantilever under a sin-ramped tip load, run to steady
+    state with the reference's convergence settings (examples/timoshenko/
+    convergence.py:25-60), against y(s) = s(-F/(aGA) + s(-FL/(2EI) + sF/(6EI)))
+    (examples/timoshenko/helpers/analytical.py:4-31): the L-inf error is a few
+    percent and halves when the element count doubles (first order)."""
+    e10, ymax = _timoshenko_error(10)
+    e20, _ = _timoshenko_error(20)
+    assert e20 <= 0.06 * ymax
+    assert 1.8 <= e10 / e20 <= 2.2, (e10, e20)
