@@ -1,0 +1,209 @@
+"""Benchmark: element-steps/s of the batched FP32-bulk Cosserat-rod step on the
+4096-rod ragged batch (n log-uniform in [16, 1024]) on one MI355X, beside the
+fused FP64 step, the plain (op-by-op) FP64 step and the CPU oracle.
+
+    python bench.py [--gpus 1] [--steps 10000] [--warmup 100]
+
+Prints ONE JSON line.  `value` is the FP32-bulk element-steps/s with the state
+resident in HBM (element-steps = sum of rod element counts x steps; ghost
+slots are not counted).  The roofline object prices the dominant kernel
+(the fused FP32-bulk step) by its algorithmic bytes B32(n) = 360 n + 72 per
+rod-step over the kernel's own duration, measured with HIP events on the
+batch stream.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "element-steps/s batched FP32-bulk vs plain FP64 at ≤1e-6 rel err; p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def b32(n):
+    return 360 * n + 72
+
+
+def b64(n):
+    return 432 * n + 48
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_block(wl):
+    from elasticapp_amd.workloads import build_block
+    return build_block(wl)
+
+
+def timed(fn, sync):
+    sync()
+    t0 = time.perf_counter()
+    fn()
+    sync()
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(wl, budget_s=12.0):
+    """The oracle (FP64 CPU restatement) on a bounded sample of the same batch:
+    the first rods of the ragged batch packed into blocks of <= 1024 elements,
+    one std::thread per host core, timed over enough steps to fill ~budget_s."""
+    from elasticapp_amd.workloads import Workload
+    from tests.gpu_helpers import oracle_system
+    threads = len(os.sched_getaffinity(0))
+    n_rods = min(512, len(wl.n_elems))
+    sub = Workload("cpu_sample", wl.n_elems[:n_rods], wl.states[:n_rods], wl.dt, 0, forces=wl.forces[:n_rods])
+    sysm = oracle_system(sub, max_block_elems=1024)
+    elems = sum(sub.n_elems)
+    t = time.perf_counter()
+    sysm.step(wl.dt, 5, threads)
+    probe = time.perf_counter() - t
+    steps = int(max(10, min(5000, budget_s / max(probe / 5, 1e-6))))
+    t = time.perf_counter()
+    sysm.step(wl.dt, steps, threads)
+    el = time.perf_counter() - t
+    cpu_model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": elems * steps / el, "unit": "element-steps/s", "cores": threads, "kind": "port",
+            "sample": "first %d rods of the batch (%d elements) in blocks of <=1024 elements, %d FP64 steps, "
+                      "%.1f s, %s" % (n_rods, elems, steps, el, cpu_model)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--chunk", type=int, default=100, help="steps per captured hipGraph replay")
+    ap.add_argument("--plain-steps", type=int, default=200, help="steps timed on the op-by-op FP64 path")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--rods", type=int, default=4096)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (rods shard by rod with no "
+              "exchange; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+    if args.steps % args.chunk or args.warmup % args.chunk:
+        print("--steps and --warmup must be multiples of --chunk", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from elasticapp_amd import FP32_BULK, FP64
+    from elasticapp_amd.workloads import ragged_c2
+
+    torch.cuda.init()
+    dev_name = torch.cuda.get_device_name(0)
+    log("device", dev_name)
+    wl = ragged_c2(n_rods=args.rods, steps=args.steps)
+    n = np.array(wl.n_elems)
+    elems = int(n.sum())
+    log("workload ragged_c2: %d rods, %d elements, dt %.3e" % (len(n), elems, wl.dt))
+
+    # ---- FP32-bulk, hipGraph chunks ----------------------------------------------------
+    blk32 = make_block(wl)
+    blk32.set_fused_config(0, args.chunk)
+
+    def sync32():
+        blk32.synchronize()
+        torch.cuda.synchronize()
+
+    blk32.step_fused_graph(args.warmup, args.chunk, FP32_BULK)
+    sync32()
+    el32 = timed(lambda: blk32.step_fused_graph(args.steps, args.chunk, FP32_BULK), sync32)
+    v32 = elems * args.steps / el32
+    log("fp32-bulk: %.3f s for %d steps -> %.3e element-steps/s" % (el32, args.steps, v32))
+    flags = blk32.check_finite()
+    assert not flags.any(), "non-finite FP32-bulk state in %d rods" % int(flags.sum())
+
+    # dominant kernel duration: one launch of `chunk` steps, HIP events on the batch stream
+    kms = []
+    for _ in range(5):
+        blk32.step_fused(args.chunk, FP32_BULK)
+        ms, _ = blk32.last_kernel_ms()
+        kms.append(ms)
+    kms = float(np.mean(kms))
+    bytes_launch = float(b32(n).sum()) * args.chunk
+    achieved = bytes_launch / (kms * 1e-3) / 1e9
+    lanes, spl = blk32.fused_config()
+
+    # ---- fused FP64 over the same steps (error reference and FP64 rate) ----------------
+    blk64 = make_block(wl)
+    blk64.set_fused_config(0, args.chunk)
+
+    def sync64():
+        blk64.synchronize()
+        torch.cuda.synchronize()
+
+    blk64.step_fused_graph(args.warmup, args.chunk, FP64)
+    sync64()
+    el64 = timed(lambda: blk64.step_fused_graph(args.steps, args.chunk, FP64), sync64)
+    v64 = elems * args.steps / el64
+    log("fp64 fused: %.3f s -> %.3e element-steps/s" % (el64, v64))
+    x32 = blk32.get_field("Position")
+    x64 = blk64.get_field("Position")
+    # exclude ghost slots (they stay at zero in both)
+    scale = np.abs(x64).max()
+    pos_err = float(np.abs(x32 - x64).max() / scale)
+    log("fp32-bulk position error vs fp64 after %d steps: %.3e" % (args.warmup + args.steps, pos_err))
+    blk32.close()
+    blk64.close()
+
+    # ---- plain FP64 (one kernel per reference operator) ----------------------------------
+    blkp = make_block(wl)
+    blkp.step(wl.dt, 5)
+    blkp.synchronize()
+    elp = timed(lambda: blkp.step(wl.dt, args.plain_steps), lambda: (blkp.synchronize(), torch.cuda.synchronize()))
+    vp = elems * args.plain_steps / elp
+    log("plain fp64: %.3f s for %d steps -> %.3e element-steps/s" % (elp, args.plain_steps, vp))
+    blkp.close()
+
+    cpu = None if args.no_cpu else cpu_baseline(wl)
+    if cpu:
+        log("cpu oracle: %.3e element-steps/s on %d threads" % (cpu["value"], cpu["cores"]))
+
+    out = {
+        "metric": METRIC,
+        "value": v32,
+        "unit": "element-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el32 * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 state)",
+        "data": "synthetic",
+        "config": {"workload": "ragged_c2: 4096 rods, n log-uniform [16,1024] (seed 2002), L U[0.5,3] m, "
+                               "r U[0.005,0.05] m, E logU[1e5,1e7], rho U[500,5000], damping U[0.01,0.5], "
+                               "N(0,0.5) rest curvature, N(0,1e-2) velocities, N(0,1) N tip forces",
+                   "rods": int(len(n)), "elements": elems, "dt": wl.dt, "graph_chunk": args.chunk,
+                   "tile_lanes": lanes, "steps_per_launch": spl, "device": dev_name},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_fused<float> (FP32-bulk)", "kernel_ms_per_launch": kms,
+                     "algorithmic_bytes_per_launch": bytes_launch},
+        "cpu_baseline": cpu,
+        "fp64_fused": {"value": v64, "ms_per_step": el64 * 1e3 / args.steps},
+        "fp64_plain": {"value": vp, "steps": args.plain_steps},
+        "ratio_fp32bulk_vs_plain_fp64": v32 / vp,
+        "fp32_bulk_position_rel_err_vs_fp64": pos_err,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,61 @@
+"""Build liberod.so (HIP kernels + C ABI) for gfx950 in-tree with hipcc.
+
+    python -m elasticapp_amd.build          # or: python elasticapp_amd/build.py
+
+The op-by-op FP64 kernels are compiled with -ffp-contract=off (every product
+and sum rounded on its own, like the CPU restatement they are checked
+against); the fused kernels may contract into FMAs.  Packed FP32 instructions
+are disabled for all device code.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "liberod.so")
+ARCH = os.environ.get("EROD_OFFLOAD_ARCH", "gfx950")
+
+COMMON = ["-O3", "-std=c++20", "-fPIC", "--offload-arch=" + ARCH,
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-Wall", "-Wno-unused-function", "-Wno-unused-result", "-Wno-unused-value"]
+SOURCES = {
+    "erod_ops.hip": ["-ffp-contract=off"],
+    "erod_fused64.hip": ["-ffp-contract=off"],
+    "erod_fused32.hip": ["-ffp-contract=fast"],
+    "erod_fused.hip": [],
+    "erod_capi.hip": [],
+}
+
+
+def hipcc():
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def build(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    cc = hipcc()
+    objs = []
+    for src, extra in SOURCES.items():
+        obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+        cmd = [cc] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    cmd = [cc, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-q" not in sys.argv))

@@ -1,0 +1,513 @@
+// C ABI of the device-resident Cosserat-rod block (include/erod.h).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "erod_common.hpp"
+
+namespace erod {
+
+  static thread_local std::string g_last_error;
+  void set_error(const std::string& msg) { g_last_error = msg; }
+
+  int sync_rods(erod_batch* b) {
+    if (!b->rods_dirty) return EROD_OK;
+    EROD_HIP(hipMemcpyAsync(b->d_rods, b->rods.data(), sizeof(RodInfo) * b->n_rods, hipMemcpyHostToDevice, b->stream));
+    EROD_HIP(hipStreamSynchronize(b->stream));
+    b->rods_dirty = false;
+    return EROD_OK;
+  }
+
+  // per-rod damping rates for the fused path: the reference stores one rate per
+  // node / element; the fused kernels need it uniform along each rod
+  static int refresh_rod_damping(erod_batch* b) {
+    const uint64_t N = b->E + 1, E = b->E;
+    std::vector<double> gf(N), gt(E);
+    EROD_HIP(hipMemcpyAsync(gf.data(), b->d_field[EROD_FORCE_DAMPING_RATE], N * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    EROD_HIP(hipMemcpyAsync(gt.data(), b->d_field[EROD_TORQUE_DAMPING_RATE], E * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    EROD_HIP(hipStreamSynchronize(b->stream));
+    for (uint64_t r = 0; r < b->n_rods; ++r) {
+      const uint64_t s = b->starts[r], n = b->n_elems[r];
+      double f = b->damped ? gf[s] : 0.0, t = b->damped ? gt[s] : 0.0;
+      if (b->damped) {
+        for (uint64_t i = 0; i <= n; ++i)
+          if (gf[s + i] != f) { set_error("fused path needs a uniform ForceDampingRate per rod"); return EROD_ERR_UNSUPPORTED; }
+        for (uint64_t i = 0; i < n; ++i)
+          if (gt[s + i] != t) { set_error("fused path needs a uniform TorqueDampingRate per rod"); return EROD_ERR_UNSUPPORTED; }
+      }
+      b->rods[r].gamma_f = f;
+      b->rods[r].gamma_t = t;
+    }
+    b->rods_dirty = true;
+    return sync_rods(b);
+  }
+
+}  // namespace erod
+
+using namespace erod;
+
+namespace {
+  bool is_fp32_const(int f) {
+    return f == EROD_MASS || f == EROD_INV_MASS || f == EROD_REFERENCE_ELEMENT_LENGTH ||
+           f == EROD_REFERENCE_SHEAR_STRETCH_STRAIN || f == EROD_SHEAR_STRETCH_RIGIDITY ||
+           f == EROD_MASS_SECOND_MOMENT_OF_INERTIA || f == EROD_INV_MASS_SECOND_MOMENT_OF_INERTIA ||
+           f == EROD_REFERENCE_VORONOI_LENGTH || f == EROD_REFERENCE_CURVATURE || f == EROD_BENDING_TWIST_RIGIDITY;
+  }
+
+  int check_batch(const erod_batch* b) {
+    if (!b) {
+      set_error("null batch");
+      return EROD_ERR_INVALID;
+    }
+    return EROD_OK;
+  }
+
+  int check_field(int f) {
+    if (f < 0 || f >= EROD_N_FIELDS) {
+      set_error("field id out of range");
+      return EROD_ERR_INVALID;
+    }
+    return EROD_OK;
+  }
+
+  void mark_field_written(erod_batch* b, int f) {
+    if (is_fp32_const(f)) b->const32_dirty = true;
+    if (f == EROD_FORCE_DAMPING_RATE || f == EROD_TORQUE_DAMPING_RATE) b->damping_dirty = true;
+  }
+}  // namespace
+
+// damping rates are re-read before the next fused call after they were written
+static int ensure_damping(erod_batch* b) {
+  if (!b->damping_dirty) return EROD_OK;
+  b->damping_dirty = false;
+  return refresh_rod_damping(b);
+}
+
+extern "C" {
+
+const char* erod_last_error(void) { return g_last_error.c_str(); }
+
+int erod_field_shape(int field, int* ncomp, int* stagger) {
+  if (check_field(field)) return EROD_ERR_INVALID;
+  if (ncomp) *ncomp = field_info(field).ncomp;
+  if (stagger) *stagger = field_info(field).stagger;
+  return EROD_OK;
+}
+
+int erod_batch_create(const uint64_t* n_elems, uint64_t n_rods, int damped, erod_batch** out) {
+  if (!out || !n_elems || n_rods == 0) {
+    set_error("erod_batch_create: need at least one rod");
+    return EROD_ERR_INVALID;
+  }
+  *out = nullptr;
+  for (uint64_t r = 0; r < n_rods; ++r)
+    if (n_elems[r] < 2) {
+      set_error("Cannot have rods with < 2 elements!");
+      return EROD_ERR_INVALID;
+    }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    set_error("no HIP device visible");
+    return EROD_ERR_NO_DEVICE;
+  }
+  auto* b = new erod_batch();
+  EROD_HIP(hipGetDevice(&b->device));
+  b->n_rods = n_rods;
+  b->damped = damped != 0;
+  b->n_elems.assign(n_elems, n_elems + n_rods);
+  uint64_t s = 0;
+  for (uint64_t r = 0; r < n_rods; ++r) {
+    b->starts.push_back(s);
+    if (r + 1 < n_rods) b->ghost_nodes.push_back(s + n_elems[r] + 1);
+    s += n_elems[r] + 2;
+  }
+  b->E = s - 2;
+  int rc = EROD_OK;
+  auto fail = [&](int code) {
+    erod_batch_destroy(b);
+    return code;
+  };
+  if ((rc = hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate"))) return fail(rc);
+  for (int f = 0; f < EROD_N_FIELDS; ++f) {
+    const size_t bytes = sizeof(double) * field_info(f).ncomp * b->field_slots(f);
+    if ((rc = hip_check(hipMalloc(&b->d_field[f], bytes), "hipMalloc field"))) return fail(rc);
+    if ((rc = hip_check(hipMemsetAsync(b->d_field[f], 0, bytes, b->stream), "hipMemset field"))) return fail(rc);
+  }
+  if (!b->ghost_nodes.empty()) {
+    if ((rc = hip_check(hipMalloc(&b->d_ghost_nodes, sizeof(uint64_t) * b->ghost_nodes.size()), "hipMalloc ghosts"))) return fail(rc);
+    if ((rc = hip_check(hipMemcpyAsync(b->d_ghost_nodes, b->ghost_nodes.data(), sizeof(uint64_t) * b->ghost_nodes.size(),
+                                       hipMemcpyHostToDevice, b->stream), "copy ghosts"))) return fail(rc);
+  }
+  b->rods.resize(n_rods);
+  for (uint64_t r = 0; r < n_rods; ++r) {
+    RodInfo& ri = b->rods[r];
+    std::memset(&ri, 0, sizeof(ri));
+    ri.start = b->starts[r];
+    ri.n = static_cast<uint32_t>(n_elems[r]);
+  }
+  if ((rc = hip_check(hipMalloc(&b->d_rods, sizeof(RodInfo) * n_rods), "hipMalloc rods"))) return fail(rc);
+  if ((rc = hip_check(hipMalloc(&b->d_time, sizeof(double) * (n_rods + 1)), "hipMalloc time"))) return fail(rc);
+  if ((rc = hip_check(hipMemsetAsync(b->d_time, 0, sizeof(double) * (n_rods + 1), b->stream), "hipMemset time"))) return fail(rc);
+  if ((rc = hip_check(hipMalloc(&b->d_flags, sizeof(int) * n_rods), "hipMalloc flags"))) return fail(rc);
+  if ((rc = hip_check(hipEventCreate(&b->ev0), "hipEventCreate"))) return fail(rc);
+  if ((rc = hip_check(hipEventCreate(&b->ev1), "hipEventCreate"))) return fail(rc);
+  if ((rc = erod_fill_ghosts(b))) return fail(rc);
+  if ((rc = hip_check(hipStreamSynchronize(b->stream), "sync"))) return fail(rc);
+  *out = b;
+  return EROD_OK;
+}
+
+int erod_batch_destroy(erod_batch* b) {
+  if (!b) return EROD_OK;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
+  for (int f = 0; f < EROD_N_FIELDS; ++f) hipFree(b->d_field[f]);
+  hipFree(b->d_ghost_nodes);
+  hipFree(b->d_rods);
+  hipFree(b->d_time);
+  hipFree(b->d_flags);
+  hipFree(b->d_const32);
+  hipFree(b->plan.d_tiles);
+  hipFree(b->plan.d_tile_rods);
+  hipFree(b->plan.d_tile_lane0);
+  if (b->ev0) hipEventDestroy(b->ev0);
+  if (b->ev1) hipEventDestroy(b->ev1);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+  return EROD_OK;
+}
+
+int erod_batch_device(const erod_batch* b, int* device) {
+  if (check_batch(b)) return EROD_ERR_INVALID;
+  *device = b->device;
+  return EROD_OK;
+}
+
+int erod_batch_slots(const erod_batch* b, uint64_t* element_slots) {
+  if (check_batch(b)) return EROD_ERR_INVALID;
+  *element_slots = b->E;
+  return EROD_OK;
+}
+
+int erod_rod_start(const erod_batch* b, uint64_t rod, uint64_t* start) {
+  if (check_batch(b)) return EROD_ERR_INVALID;
+  if (rod >= b->n_rods) {
+    set_error("rod index out of range");
+    return EROD_ERR_INVALID;
+  }
+  *start = b->starts[rod];
+  return EROD_OK;
+}
+
+int erod_upload_field(erod_batch* b, int field, const double* host) {
+  if (check_batch(b) || check_field(field) || !host) return EROD_ERR_INVALID;
+  const size_t bytes = sizeof(double) * field_info(field).ncomp * b->field_slots(field);
+  EROD_HIP(hipMemcpyAsync(b->d_field[field], host, bytes, hipMemcpyHostToDevice, b->stream));
+  EROD_HIP(hipStreamSynchronize(b->stream));
+  mark_field_written(b, field);
+  return EROD_OK;
+}
+
+int erod_download_field(erod_batch* b, int field, double* host) {
+  if (check_batch(b) || check_field(field) || !host) return EROD_ERR_INVALID;
+  const size_t bytes = sizeof(double) * field_info(field).ncomp * b->field_slots(field);
+  EROD_HIP(hipMemcpyAsync(host, b->d_field[field], bytes, hipMemcpyDeviceToHost, b->stream));
+  EROD_HIP(hipStreamSynchronize(b->stream));
+  return EROD_OK;
+}
+
+static uint64_t rod_count(const erod_batch* b, int field, uint64_t rod) {
+  const int st = field_info(field).stagger;
+  const uint64_t n = b->n_elems[rod];
+  return st == EROD_NODE ? n + 1 : (st == EROD_ELEMENT ? n : n - 1);
+}
+
+int erod_upload_rod(erod_batch* b, int field, uint64_t rod, const double* host) {
+  if (check_batch(b) || check_field(field) || !host || rod >= b->n_rods) {
+    set_error("erod_upload_rod: bad argument");
+    return EROD_ERR_INVALID;
+  }
+  const uint64_t cnt = rod_count(b, field, rod), slots = b->field_slots(field), s = b->starts[rod];
+  const int nc = field_info(field).ncomp;
+  // 2D copy: nc lines of cnt values, pitch = slots
+  EROD_HIP(hipMemcpy2DAsync(b->d_field[field] + s, slots * sizeof(double), host, cnt * sizeof(double), cnt * sizeof(double),
+                            nc, hipMemcpyHostToDevice, b->stream));
+  EROD_HIP(hipStreamSynchronize(b->stream));
+  mark_field_written(b, field);
+  return EROD_OK;
+}
+
+int erod_download_rod(erod_batch* b, int field, uint64_t rod, double* host) {
+  if (check_batch(b) || check_field(field) || !host || rod >= b->n_rods) {
+    set_error("erod_download_rod: bad argument");
+    return EROD_ERR_INVALID;
+  }
+  const uint64_t cnt = rod_count(b, field, rod), slots = b->field_slots(field), s = b->starts[rod];
+  const int nc = field_info(field).ncomp;
+  EROD_HIP(hipMemcpy2DAsync(host, cnt * sizeof(double), b->d_field[field] + s, slots * sizeof(double), cnt * sizeof(double),
+                            nc, hipMemcpyDeviceToHost, b->stream));
+  EROD_HIP(hipStreamSynchronize(b->stream));
+  return EROD_OK;
+}
+
+int erod_fill_ghosts(erod_batch* b) {
+  if (check_batch(b)) return EROD_ERR_INVALID;
+  for (int f = 0; f < EROD_N_FIELDS; ++f) {
+    const int rc = ops_fill_ghosts(b, f);
+    if (rc) return rc;
+  }
+  return EROD_OK;
+}
+
+int erod_kinematic_update(erod_batch* b, double dt) {
+  if (check_batch(b)) return EROD_ERR_INVALID;
+  return ops_kinematic_update(b, dt);
+}
+
+int erod_compute_internal_dynamics(erod_batch* b, double time) {
+  (void)time;
+  if (check_batch(b)) return EROD_ERR_INVALID;
+  return ops_compute_internal_dynamics(b);
+}
+
+int erod_dynamic_update(erod_batch* b, double dt) {
+  if (check_batch(b)) return EROD_ERR_INVALID;
+  return ops_dynamic_update(b, dt);
+}
+
+int erod_add_clamp(erod_batch* b, uint64_t rod, uint64_t index, const double x[3], const double Q[9]) {
+  if (check_batch(b) || rod >= b->n_rods || !x || !Q) {
+    set_error("erod_add_clamp: bad argument");
+    return EROD_ERR_INVALID;
+  }
+  if (index != 0) {
+    set_error("erod_add_clamp: only index 0 (the rod's first node / element) is supported");
+    return EROD_ERR_UNSUPPORTED;
+  }
+  RodInfo& ri = b->rods[rod];
+  ri.flags |= 1u;
+  std::memcpy(ri.clamp_x, x, sizeof(ri.clamp_x));
+  std::memcpy(ri.clamp_q, Q, sizeof(ri.clamp_q));
+  b->has_clamps = true;
+  b->rods_dirty = true;
+  b->graph_precision = -1;
+  return EROD_OK;
+}
+
+int erod_add_end_force(erod_batch* b, uint64_t rod, const double F[3], int ramp, double start, double interval) {
+  if (check_batch(b) || rod >= b->n_rods || !F || ramp < 0 || ramp > 2) {
+    set_error("erod_add_end_force: bad argument");
+    return EROD_ERR_INVALID;
+  }
+  RodInfo& ri = b->rods[rod];
+  ri.flags |= 2u;
+  std::memcpy(ri.force, F, sizeof(ri.force));
+  ri.ramp = ramp;
+  ri.ramp_start = start;
+  ri.ramp_interval = interval;
+  b->has_forces = true;
+  b->rods_dirty = true;
+  b->graph_precision = -1;
+  return EROD_OK;
+}
+
+int erod_set_gravity(erod_batch* b, const double g[3]) {
+  if (check_batch(b) || !g) return EROD_ERR_INVALID;
+  b->env.gravity = 1;
+  std::memcpy(b->env.g, g, sizeof(b->env.g));
+  b->graph_precision = -1;
+  return EROD_OK;
+}
+
+int erod_set_plane_contact(erod_batch* b, double k, double c, double mu, double ct, double z0, double offset) {
+  if (check_batch(b)) return EROD_ERR_INVALID;
+  b->env.contact = 1;
+  b->env.k = k;
+  b->env.c = c;
+  b->env.mu = mu;
+  b->env.ct = ct;
+  b->env.z0 = z0;
+  b->env.offset = offset;
+  b->graph_precision = -1;
+  return EROD_OK;
+}
+
+int erod_apply_post_kinematic(erod_batch* b, double time) {
+  (void)time;
+  if (check_batch(b)) return EROD_ERR_INVALID;
+  int rc = sync_rods(b);
+  return rc ? rc : ops_apply_post_kinematic(b);
+}
+
+int erod_apply_pre_dynamic(erod_batch* b, double time) {
+  if (check_batch(b)) return EROD_ERR_INVALID;
+  int rc = sync_rods(b);
+  return rc ? rc : ops_apply_pre_dynamic(b, time);
+}
+
+int erod_apply_post_dynamic(erod_batch* b, double time) {
+  (void)time;
+  if (check_batch(b)) return EROD_ERR_INVALID;
+  int rc = sync_rods(b);
+  return rc ? rc : ops_apply_post_dynamic(b);
+}
+
+int erod_step(erod_batch* b, double* time, double dt, uint64_t n) {
+  if (check_batch(b) || !time) return EROD_ERR_INVALID;
+  int rc = sync_rods(b);
+  if (rc) return rc;
+  double t = *time;
+  for (uint64_t i = 0; i < n; ++i) {
+    if ((rc = ops_kinematic_update(b, dt))) return rc;
+    t += 0.5 * dt;
+    if ((rc = ops_apply_post_kinematic(b))) return rc;
+    if ((rc = ops_compute_internal_dynamics(b))) return rc;
+    if ((rc = ops_apply_pre_dynamic(b, t))) return rc;
+    if ((rc = ops_dynamic_update(b, dt))) return rc;
+    if ((rc = ops_apply_post_dynamic(b))) return rc;
+    t += 0.5 * dt;
+    if ((rc = ops_kinematic_update(b, dt))) return rc;
+  }
+  *time = t;
+  return EROD_OK;
+}
+
+int erod_set_rod_dt(erod_batch* b, const double* dt_per_rod) {
+  if (check_batch(b) || !dt_per_rod) return EROD_ERR_INVALID;
+  for (uint64_t r = 0; r < b->n_rods; ++r) {
+    if (!(dt_per_rod[r] > 0.0) || !std::isfinite(dt_per_rod[r])) {
+      set_error("erod_set_rod_dt: dt must be positive and finite");
+      return EROD_ERR_INVALID;
+    }
+    b->rods[r].dt = dt_per_rod[r];
+  }
+  b->rods_dirty = true;
+  return EROD_OK;
+}
+
+int erod_get_rod_time(erod_batch* b, double* t) {
+  if (check_batch(b) || !t) return EROD_ERR_INVALID;
+  EROD_HIP(hipMemcpyAsync(t, b->d_time, sizeof(double) * b->n_rods, hipMemcpyDeviceToHost, b->stream));
+  EROD_HIP(hipStreamSynchronize(b->stream));
+  return EROD_OK;
+}
+
+int erod_set_rod_time(erod_batch* b, const double* t) {
+  if (check_batch(b) || !t) return EROD_ERR_INVALID;
+  EROD_HIP(hipMemcpyAsync(b->d_time, t, sizeof(double) * b->n_rods, hipMemcpyHostToDevice, b->stream));
+  EROD_HIP(hipStreamSynchronize(b->stream));
+  return EROD_OK;
+}
+
+int erod_set_fused_config(erod_batch* b, int lanes_per_tile, int steps_per_launch) {
+  if (check_batch(b) || steps_per_launch < 0) return EROD_ERR_INVALID;
+  b->plan.lanes_per_tile = lanes_per_tile;
+  b->plan.steps_per_launch = steps_per_launch;
+  b->plan.valid = false;
+  b->graph_precision = -1;
+  return EROD_OK;
+}
+
+int erod_get_fused_config(const erod_batch* b, int* lanes, int* steps) {
+  if (check_batch(b)) return EROD_ERR_INVALID;
+  if (lanes) *lanes = b->plan.chosen_lanes;
+  if (steps) *steps = b->plan.steps_per_launch > 0 ? b->plan.steps_per_launch : 100;
+  return EROD_OK;
+}
+
+static int fused_common(erod_batch* b, int precision) {
+  if (precision != EROD_FP64 && precision != EROD_FP32_BULK) {
+    set_error("unknown precision");
+    return EROD_ERR_INVALID;
+  }
+  for (uint64_t r = 0; r < b->n_rods; ++r)
+    if (!(b->rods[r].dt > 0.0)) {
+      set_error("fused path: set a positive dt for every rod with erod_set_rod_dt");
+      return EROD_ERR_INVALID;
+    }
+  int rc = ensure_damping(b);
+  if (rc) return rc;
+  return fused_prepare(b, precision);
+}
+
+int erod_step_fused(erod_batch* b, int precision, uint64_t n) {
+  if (check_batch(b)) return EROD_ERR_INVALID;
+  int rc = fused_common(b, precision);
+  if (rc) return rc;
+  const uint64_t spl = b->plan.steps_per_launch > 0 ? uint64_t(b->plan.steps_per_launch) : 100;
+  uint64_t done = 0;
+  b->last_launches = 0;
+  while (done < n) {
+    const uint64_t s = std::min<uint64_t>(spl, n - done);
+    const bool timed = done == 0;
+    if ((rc = fused_launch(b, precision, static_cast<int>(s), timed))) return rc;
+    ++b->last_launches;
+    done += s;
+  }
+  return EROD_OK;
+}
+
+int erod_step_fused_graph(erod_batch* b, int precision, uint64_t n, uint64_t chunk) {
+  if (check_batch(b) || chunk == 0 || n % chunk != 0) {
+    set_error("erod_step_fused_graph: n must be a positive multiple of chunk");
+    return EROD_ERR_INVALID;
+  }
+  int rc = fused_common(b, precision);
+  if (rc) return rc;
+  const uint64_t spl = b->plan.steps_per_launch > 0 ? uint64_t(b->plan.steps_per_launch) : 100;
+  if (b->graph_precision != precision || b->graph_chunk != chunk || !b->graph_exec) {
+    if (b->graph_exec) {
+      hipGraphExecDestroy(b->graph_exec);
+      b->graph_exec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    EROD_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    uint64_t done = 0;
+    int lrc = EROD_OK;
+    while (done < chunk && lrc == EROD_OK) {
+      const uint64_t s = std::min<uint64_t>(spl, chunk - done);
+      lrc = fused_launch(b, precision, static_cast<int>(s), false);
+      done += s;
+    }
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (lrc) return lrc;
+    EROD_HIP(e);
+    EROD_HIP(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+    hipGraphDestroy(g);
+    b->graph_precision = precision;
+    b->graph_chunk = chunk;
+  }
+  for (uint64_t i = 0; i < n / chunk; ++i) EROD_HIP(hipGraphLaunch(b->graph_exec, b->stream));
+  return EROD_OK;
+}
+
+int erod_reorthonormalize(erod_batch* b) {
+  if (check_batch(b)) return EROD_ERR_INVALID;
+  return fused_reorthonormalize(b);
+}
+
+int erod_last_kernel_ms(const erod_batch* b, double* ms, uint64_t* launches) {
+  if (check_batch(b)) return EROD_ERR_INVALID;
+  float f = 0.0f;
+  EROD_HIP(hipEventSynchronize(b->ev1));
+  EROD_HIP(hipEventElapsedTime(&f, b->ev0, b->ev1));
+  if (ms) *ms = f;
+  if (launches) *launches = b->last_launches;
+  return EROD_OK;
+}
+
+int erod_synchronize(erod_batch* b) {
+  if (check_batch(b)) return EROD_ERR_INVALID;
+  EROD_HIP(hipStreamSynchronize(b->stream));
+  return EROD_OK;
+}
+
+int erod_check_finite(erod_batch* b, uint8_t* rod_flags) {
+  if (check_batch(b) || !rod_flags) return EROD_ERR_INVALID;
+  return fused_check_finite(b, rod_flags);
+}
+
+}  // extern "C"

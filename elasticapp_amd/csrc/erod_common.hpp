@@ -1,0 +1,169 @@
+// Shared definitions of the device-resident Cosserat-rod block: field table,
+// ghost values, batch state and launch helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/erod.h"
+
+namespace erod {
+
+  struct FieldInfo {
+    int ncomp;
+    int stagger;   // erod_stagger
+    double ghost;  // value of ghost slots; < -1 marks the director ghost
+  };
+
+  // Ghost values of the reference variables: scalars {} = 0
+  // (common/Traits/DataType/ScalarTag.hpp:85), vectors 0 (MatrixTag.hpp:102),
+  // lengths / volume / dimension / mass 1 (CosseratRodSpanwiseGeometry.hpp:97,153,179,
+  // CosseratRodCrossSectionInterface.hpp:95,121, WithRodKinematics.hpp:101),
+  // director {{.8,.6,0},{0,0,1},{.6,-.8,0}} (RodSpanwiseGeometry.hpp:96-101).
+  inline const FieldInfo& field_info(int f) {
+    static const FieldInfo t[EROD_N_FIELDS] = {
+        {3, EROD_NODE, 0.0},     {3, EROD_NODE, 0.0},    {3, EROD_NODE, 0.0},
+        {1, EROD_NODE, 1.0},     {1, EROD_NODE, 0.0},    {3, EROD_NODE, 0.0},
+        {3, EROD_NODE, 0.0},     {1, EROD_NODE, 0.0},
+        {9, EROD_ELEMENT, -2.0}, {3, EROD_ELEMENT, 0.0}, {3, EROD_ELEMENT, 0.0},
+        {1, EROD_ELEMENT, 1.0},  {1, EROD_ELEMENT, 1.0}, {1, EROD_ELEMENT, 0.0},
+        {3, EROD_ELEMENT, 0.0},  {1, EROD_ELEMENT, 1.0}, {1, EROD_ELEMENT, 1.0},
+        {3, EROD_ELEMENT, 0.0},  {3, EROD_ELEMENT, 0.0}, {3, EROD_ELEMENT, 0.0},
+        {3, EROD_ELEMENT, 0.0},  {3, EROD_ELEMENT, 0.0}, {3, EROD_ELEMENT, 0.0},
+        {3, EROD_ELEMENT, 0.0},  {3, EROD_ELEMENT, 0.0}, {1, EROD_ELEMENT, 0.0},
+        {3, EROD_ELEMENT, 0.0},  {3, EROD_ELEMENT, 0.0},
+        {1, EROD_VORONOI, 1.0},  {1, EROD_VORONOI, 0.0}, {1, EROD_VORONOI, 0.0},
+        {3, EROD_VORONOI, 0.0},  {3, EROD_VORONOI, 0.0}, {3, EROD_VORONOI, 0.0},
+        {3, EROD_VORONOI, 0.0},  {3, EROD_VORONOI, 0.0},
+    };
+    return t[f];
+  }
+
+  // Rod descriptor used by the fused kernels and the hooks (device copy).
+  struct RodInfo {
+    uint64_t start;   // slot of node 0 / element 0 / voronoi 0
+    uint32_t n;       // elements
+    uint32_t flags;   // bit0 clamp at index 0, bit1 end force
+    double dt;        // timestep of this rod (fused path)
+    double gamma_f;   // force damping rate (uniform per rod)
+    double gamma_t;   // torque damping rate (uniform per rod)
+    double clamp_x[3];
+    double clamp_q[9];
+    double force[3];
+    int ramp;
+    double ramp_start, ramp_interval;
+  };
+
+  struct Env {
+    int gravity;
+    double g[3];
+    int contact;
+    double k, c, mu, ct, z0, offset;
+  };
+
+  // A tile: the rods one workgroup of the fused kernel owns.
+  struct Tile {
+    uint32_t first;   // index into the tile rod list
+    uint32_t count;   // rods in this tile
+    uint32_t lanes;   // elements (lanes) used
+    uint32_t pad;
+  };
+
+  struct FusedPlan {
+    int lanes_per_tile = 0;       // requested (0 = autotune)
+    int steps_per_launch = 0;     // requested (0 = autotune)
+    int chosen_lanes = 0;
+    int chosen_steps = 0;
+    int epl = 1;                  // elements per lane
+    uint64_t n_tiles = 0;
+    uint32_t* d_tile_rods = nullptr;  // rod ids in tile order
+    uint32_t* d_tile_lane0 = nullptr; // first lane of each rod within its tile
+    Tile* d_tiles = nullptr;
+    bool valid = false;
+  };
+
+}  // namespace erod
+
+struct erod_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t n_rods = 0;
+  uint64_t E = 0;  // element slots incl. ghosts
+  bool damped = true;
+  std::vector<uint64_t> n_elems, starts, ghost_nodes;
+  double* d_field[EROD_N_FIELDS] = {};
+  uint64_t* d_ghost_nodes = nullptr;
+  // rods, hooks, clocks
+  std::vector<erod::RodInfo> rods;
+  erod::RodInfo* d_rods = nullptr;
+  double* d_time = nullptr;  // per-rod clock
+  bool rods_dirty = true;
+  bool damping_dirty = true;  // per-rod damping rates must be re-read for the fused path
+  bool has_clamps = false, has_forces = false;
+  erod::Env env{};
+  // FP32 copies of the constants for the FP32-bulk path
+  float* d_const32 = nullptr;
+  uint64_t const32_floats = 0;
+  bool const32_dirty = true;
+  erod::FusedPlan plan;
+  // graph cache
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_precision = -1;
+  uint64_t graph_chunk = 0;
+  // timing of the dominant kernel
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_kernel_ms = 0.0;
+  uint64_t last_launches = 0;
+  // scratch
+  int* d_flags = nullptr;
+
+  uint64_t slots(int stagger) const { return stagger == EROD_NODE ? E + 1 : (stagger == EROD_ELEMENT ? E : E - 1); }
+  uint64_t field_slots(int f) const { return slots(erod::field_info(f).stagger); }
+};
+
+namespace erod {
+
+  void set_error(const std::string& msg);
+
+  inline int hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+      set_error(std::string(what) + ": " + hipGetErrorString(e));
+      return EROD_ERR_HIP;
+    }
+    return EROD_OK;
+  }
+
+#define EROD_HIP(call)                                   \
+  do {                                                   \
+    int _rc = ::erod::hip_check((call), #call);          \
+    if (_rc != EROD_OK) return _rc;                      \
+  } while (0)
+
+  constexpr int kThreads = 256;
+  inline unsigned grid_for(uint64_t n) {
+    uint64_t g = (n + kThreads - 1) / kThreads;
+    if (g == 0) g = 1;
+    if (g > 65535ull * 16) g = 65535ull * 16;
+    return static_cast<unsigned>(g);
+  }
+
+  // op-by-op path (erod_ops.hip)
+  int ops_kinematic_update(erod_batch* b, double dt);
+  int ops_compute_internal_dynamics(erod_batch* b);
+  int ops_dynamic_update(erod_batch* b, double dt);
+  int ops_fill_ghosts(erod_batch* b, int field);
+  int ops_apply_post_kinematic(erod_batch* b);
+  int ops_apply_pre_dynamic(erod_batch* b, double time);
+  int ops_apply_post_dynamic(erod_batch* b);
+  int ops_zero_external(erod_batch* b);
+  // fused path (erod_fused.hip)
+  int fused_prepare(erod_batch* b, int precision);
+  int fused_launch(erod_batch* b, int precision, int steps, bool timed);
+  int fused_reorthonormalize(erod_batch* b);
+  int fused_check_finite(erod_batch* b, uint8_t* flags);
+  int sync_rods(erod_batch* b);
+
+}  // namespace erod

@@ -1,0 +1,630 @@
+#pragma once
+// Fused, persistent position-Verlet step kernel (device code, instantiated
+// for FP64 in erod_fused64.hip and for FP32-bulk in erod_fused32.hip).
+//
+// One workgroup owns a tile of whole rods.  Each thread holds P contiguous
+// elements of the tile (element k, node k and, for the last element of a rod,
+// the rod's last node n) with the state x, v, Q, w in FP64 registers across
+// `nsteps` steps; only the constants are re-read each step (they stay in L2).
+// The only cross-thread data of a step are the +1/+2 neighbour node values,
+// the +1 neighbour director (first exchange) and the -1 neighbour element
+// force / couple terms (second exchange), passed through two LDS arrays, so a
+// step costs two workgroup barriers and no HBM traffic for the state.
+//
+// Arithmetic of one step follows the reference block timestep
+// (PositionVerlet.hpp:49-88, SymplecticSteppers.hpp:152-216) with explicit rod
+// ends in place of the ghost slots (zero ghosts make the block-wide difference
+// and quadrature operators reduce to the per-rod boundary rules used here):
+//   geometry/strain   CosseratRodSpanwiseGeometry.hpp:578-664
+//   loads             WithDiagonalLinearHyperElasticModel.hpp:197-238
+//   torques           WithDiagonalLinearHyperElasticModel.hpp:246-391
+//   damping           ExplicitDampingAdapterFacade.hpp:177-235
+//   accelerations     WithRodKinematics.hpp:683-714
+//   director update   SO3PrimitiveAddAssign/Scalar.hpp:25-96
+//
+// Precision: R = double keeps the reference formulas (acos-based log map with
+// its 1e-10 / 1e-14 offsets), its divisions and its order of evaluation, and
+// is compiled without FMA contraction, so that it rounds like the reference.  R = float is the FP32-bulk path: FP64 state,
+// FP64 position/velocity/angular-velocity accumulation and FP64 exponential
+// map; strains, stresses, couples, loads and torques in FP32, with the
+// cancellation-prone parts (node differences, the relative rotation of
+// neighbouring directors) formed in FP64 before rounding, and the log map
+// rewritten as theta = 2 asin(sqrt(sin^2(theta/2) + 5e-11)), which is the
+// reference's acos(cos(theta) - 1e-10) without its cancellation.
+#include "erod_common.hpp"
+
+
+namespace erod {
+  namespace fused {
+
+    template <typename R>
+    struct Consts {
+      const R* m;     // node
+      const R* invm;  // node
+      const R* l0;    // element
+      const R* sig0;  // element, 3 lines
+      const R* S;     // element, 3 lines
+      const R* J;     // element, 3 lines
+      const R* iJ;    // element, 3 lines
+      const R* D0;    // voronoi
+      const R* kap0;  // voronoi, 3 lines
+      const R* B;     // voronoi, 3 lines
+    };
+
+    // ---- small helpers ---------------------------------------------------------
+
+    __device__ __forceinline__ float rcp_(float x) { return __builtin_amdgcn_rcpf(x); }
+    __device__ __forceinline__ double rcp_(double x) { return 1.0 / x; }
+    __device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
+    __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
+    __device__ __forceinline__ float div_(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    __device__ __forceinline__ double div_(double a, double b) { return a / b; }
+    // element length from an FP64 node difference; both neighbours of a
+    // Voronoi region evaluate it the same way
+    template <typename R>
+    __device__ __forceinline__ R len_(double d0, double d1, double d2) {
+      if constexpr (sizeof(R) == 8) return sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+      else return sqrt_(static_cast<float>(d0 * d0 + d1 * d1 + d2 * d2));
+    }
+
+    // sin / cos of a rotation angle in FP64.  The director increments of one
+    // half step are tiny (|w| dt / 2), where the Taylor series to x^9 / x^8 is
+    // exact to rounding; larger angles use the library functions.
+    __device__ __forceinline__ void sincos_small(double th, double& s, double& c) {
+      if (th < 1.0e-2) {
+        const double t2 = th * th;
+        s = th * (1.0 - t2 * (1.0 / 6.0) * (1.0 - t2 * (1.0 / 20.0) * (1.0 - t2 * (1.0 / 42.0) * (1.0 - t2 * (1.0 / 72.0)))));
+        c = 1.0 - t2 * 0.5 * (1.0 - t2 * (1.0 / 12.0) * (1.0 - t2 * (1.0 / 30.0) * (1.0 - t2 * (1.0 / 56.0))));
+      } else {
+        sincos(th, &s, &c);
+      }
+    }
+
+    // Q <- exp(-[u]x) Q with u = c w (SO3AddAssign, Scalar.hpp:25-96), FP64.
+    // The FP64 path (R = double) calls the library sin / cos like the
+    // reference; the FP32-bulk path uses the small-angle series.
+    template <typename R>
+    __device__ __forceinline__ void rodrigues(double q[9], const double w[3], double cdt) {
+      double u0 = cdt * w[0], u1 = cdt * w[1], u2 = cdt * w[2];
+      const double theta = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
+      double beta, c_alpha;
+      if constexpr (sizeof(R) == 8) {
+        beta = sin(theta);
+        c_alpha = cos(theta);
+      } else {
+        sincos_small(theta, beta, c_alpha);
+      }
+      const double alpha = 1.0 - c_alpha;
+      const double den = theta + 1e-14;
+      u0 /= den;
+      u1 /= den;
+      u2 /= den;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double a = q[j], b = q[3 + j], cc = q[6 + j];
+        const double com = a * u0 + b * u1 + cc * u2;
+        q[j] = c_alpha * a + alpha * u0 * com + beta * (b * u2 - cc * u1);
+        q[3 + j] = c_alpha * b + alpha * u1 * com + beta * (cc * u0 - a * u2);
+        q[6 + j] = c_alpha * cc + alpha * u2 * com + beta * (a * u1 - b * u0);
+      }
+    }
+
+    template <typename R>
+    __device__ __forceinline__ void cross3(const R a[3], const R b[3], R o[3]) {
+      o[0] = a[1] * b[2] - a[2] * b[1];
+      o[1] = a[2] * b[0] - a[0] * b[2];
+      o[2] = a[0] * b[1] - a[1] * b[0];
+    }
+
+    __device__ __forceinline__ double ramp_value(int kind, double t, double start, double interval) {
+      if (kind == EROD_RAMP_NONE) return 1.0;
+      if (t < start) return 0.0;
+      if (t > start + interval) return 1.0;
+      if (kind == EROD_RAMP_LINEAR) return (t - start) / interval;
+      const double s = 2.0 * ((t - start) / interval) - 1.0;
+      return 0.5 * (s + (1.0 + (sin(M_PI * s) / M_PI)));
+    }
+
+    // curvature of the Voronoi region between directors qc and qn, divided by D0
+    template <typename R>
+    __device__ __forceinline__ void curvature(const double qc[9], const double qn[9], R D0, R kap[3]);
+
+    // reference formula (InvRotateDivide/Scalar.hpp:76-103)
+    template <>
+    __device__ __forceinline__ void curvature<double>(const double qc[9], const double qn[9], double D0, double kap[3]) {
+      const double tr = qc[0] * qn[0] + qc[1] * qn[1] + qc[2] * qn[2] + qc[3] * qn[3] + qc[4] * qn[4] +
+                        qc[5] * qn[5] + qc[6] * qn[6] + qc[7] * qn[7] + qc[8] * qn[8];
+      const double theta = acos(0.5 * tr - (0.5 + 1e-10));
+      const double st = 0.5 * theta / sin(theta + 1e-14) / D0;
+      kap[0] = st * (-qc[3] * qn[6] - qc[4] * qn[7] - qc[5] * qn[8] + qc[6] * qn[3] + qc[7] * qn[4] + qc[8] * qn[5]);
+      kap[1] = st * (qc[0] * qn[6] + qc[1] * qn[7] + qc[2] * qn[8] - qc[6] * qn[0] - qc[7] * qn[1] - qc[8] * qn[2]);
+      kap[2] = st * (-qc[0] * qn[3] - qc[1] * qn[4] - qc[2] * qn[5] + qc[3] * qn[0] + qc[4] * qn[1] + qc[5] * qn[2]);
+    }
+
+    // FP32-bulk: the same quantity without the cancellation of acos near 1
+    template <>
+    __device__ __forceinline__ void curvature<float>(const double qc[9], const double qn[9], float D0, float kap[3]) {
+      const double tr = qc[0] * qn[0] + qc[1] * qn[1] + qc[2] * qn[2] + qc[3] * qn[3] + qc[4] * qn[4] +
+                        qc[5] * qn[5] + qc[6] * qn[6] + qc[7] * qn[7] + qc[8] * qn[8];
+      // 2 sin(theta) * axis
+      const double s0 = -qc[3] * qn[6] - qc[4] * qn[7] - qc[5] * qn[8] + qc[6] * qn[3] + qc[7] * qn[4] + qc[8] * qn[5];
+      const double s1 = qc[0] * qn[6] + qc[1] * qn[7] + qc[2] * qn[8] - qc[6] * qn[0] - qc[7] * qn[1] - qc[8] * qn[2];
+      const double s2 = -qc[0] * qn[3] - qc[1] * qn[4] - qc[2] * qn[5] + qc[3] * qn[0] + qc[4] * qn[1] + qc[5] * qn[2];
+      // sin^2(theta/2) = (3 - tr) / 4; the reference's -1e-10 inside acos adds 5e-11
+      float a = static_cast<float>((3.0 - tr) * 0.25) + 5e-11f;
+      a = fminf(fmaxf(a, 0.0f), 1.0f);
+      const float theta = 2.0f * asinf(sqrt_(a));
+      const float st = div_(0.5f * theta, sinf(theta + 1e-14f) * D0);
+      kap[0] = st * static_cast<float>(s0);
+      kap[1] = st * static_cast<float>(s1);
+      kap[2] = st * static_cast<float>(s2);
+    }
+
+    // per-element values carried from the first to the second half of a step
+    template <typename R>
+    struct Mid {
+      R c[3];      // Q^T n / e
+      R P[3];      // M / eps^3 (Voronoi k; 0 at the rod end)
+      R Rv[3];     // (kappa x M) D0 / eps^3 (Voronoi k; 0 at the rod end)
+      R shear[3];  // (Q t x n) l0
+      R lag[3];    // (J w / e) x w
+      R unst[3];   // (J w / e) (t . dv) / (e l0)
+      R e;         // dilatation
+    };
+
+    // Element k (and Voronoi k when it exists) of a rod: everything that needs
+    // only the +1 / +2 neighbours.
+    template <typename R>
+    __device__ __forceinline__ void element_forces(const double xk[3], const double xk1[3], const double xk2[3],
+                                                   const double vk[3], const double vk1[3], const double qk[9],
+                                                   const double qk1[9], const double wk[3], bool has_vor,
+                                                   const Consts<R>& C, uint64_t es, uint64_t E, Mid<R>& out) {
+      const uint64_t V = E - 1;
+      // spanwise geometry: node difference in FP64, then R
+      const double d0 = xk1[0] - xk[0], d1 = xk1[1] - xk[1], d2 = xk1[2] - xk[2];
+      const R l = len_<R>(d0, d1, d2);
+      R t[3];
+      if constexpr (sizeof(R) == 8) {
+        t[0] = d0 / l;
+        t[1] = d1 / l;
+        t[2] = d2 / l;
+      } else {
+        const float il = rcp_(l);
+        t[0] = static_cast<float>(d0) * il;
+        t[1] = static_cast<float>(d1) * il;
+        t[2] = static_cast<float>(d2) * il;
+      }
+      const R l0 = C.l0[es];
+      const R e = div_(l, l0);
+      // shear / stretch strain sigma = e (Q t) - z; Q t formed in FP64
+      R qt[3];
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+        qt[p] = static_cast<R>(qk[3 * p] * static_cast<double>(t[0]) + qk[3 * p + 1] * static_cast<double>(t[1]) +
+                               qk[3 * p + 2] * static_cast<double>(t[2]));
+      R n[3];
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        R sig = qt[p] * e;
+        if (p == 2) sig -= R(1);
+        n[p] = C.S[p * E + es] * (sig - C.sig0[p * E + es]);
+      }
+      // Q^T n / e
+      R qn[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        qn[j] = static_cast<R>(qk[j] * static_cast<double>(n[0]) + qk[3 + j] * static_cast<double>(n[1]) +
+                               qk[6 + j] * static_cast<double>(n[2]));
+#pragma unroll
+      for (int j = 0; j < 3; ++j) out.c[j] = div_(qn[j], e);
+      out.e = e;
+      // local torque terms: (Q t x n) l0 + (J w / e) x w + (J w / e) (t . dv) / (e l0)
+      R sh[3];
+      cross3(qt, n, sh);
+      R w[3] = {static_cast<R>(wk[0]), static_cast<R>(wk[1]), static_cast<R>(wk[2])};
+      R jw[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) jw[c] = div_(C.J[c * E + es] * w[c], e);
+      R lag[3];
+      cross3(jw, w, lag);
+      const R dv0 = static_cast<R>(vk1[0] - vk[0]), dv1 = static_cast<R>(vk1[1] - vk[1]), dv2 = static_cast<R>(vk1[2] - vk[2]);
+      const R rate = div_(t[0] * dv0 + t[1] * dv1 + t[2] * dv2, e * l0);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        out.shear[c] = sh[c] * l0;
+        out.lag[c] = lag[c];
+        out.unst[c] = jw[c] * rate;
+      }
+      // Voronoi k
+      if (has_vor) {
+        const double f0 = xk2[0] - xk1[0], f1 = xk2[1] - xk1[1], f2 = xk2[2] - xk1[2];
+        const R l1 = len_<R>(f0, f1, f2);
+        const R D0 = C.D0[es];
+        const R D = R(0.5) * (l + l1);
+        const R eps = div_(D, D0);
+        R kap[3];
+        curvature<R>(qk, qk1, D0, kap);
+        R M[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) M[c] = C.B[c * V + es] * (kap[c] - C.kap0[c * V + es]);
+        const R ic = rcp_(eps * eps * eps);
+        R km[3];
+        cross3(kap, M, km);
+        const R icd = ic * D0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          out.P[c] = M[c] * ic;
+          out.Rv[c] = km[c] * icd;
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          out.P[c] = R(0);
+          out.Rv[c] = R(0);
+        }
+      }
+    }
+
+    // external load on a node: gravity and per-node plane contact
+    __device__ __forceinline__ void node_environment(const Env& env, double m, const double x[3], const double v[3], double fe[3]) {
+      if (env.gravity)
+        for (int c = 0; c < 3; ++c) fe[c] += env.g[c] * m;
+      if (env.contact) {
+        const double gap = x[2] - env.z0 - env.offset;
+        if (gap < 0.0) {
+          const double delta = -gap;
+          const double vn = -v[2];
+          const double fn = fmax(0.0, env.k * delta + env.c * vn);
+          const double vtn = sqrt(v[0] * v[0] + v[1] * v[1]);
+          fe[2] += fn;
+          if (vtn > 0.0) {
+            const double ft = fmin(env.ct * vtn, env.mu * fn);
+            fe[0] -= ft * (v[0] / vtn);
+            fe[1] -= ft * (v[1] / vtn);
+          }
+        }
+      }
+    }
+
+    template <typename R, int P, int NT>
+    __global__ void __launch_bounds__(NT) k_fused(double* __restrict__ X, double* __restrict__ Vel, double* __restrict__ Qd,
+                                                  double* __restrict__ W, Consts<R> C, const RodInfo* __restrict__ rods,
+                                                  const Tile* __restrict__ tiles, const uint32_t* __restrict__ tile_rods,
+                                                  const uint32_t* __restrict__ tile_lane0, double* __restrict__ rod_time,
+                                                  Env env, int nsteps, uint64_t E) {
+      const uint64_t N = E + 1;
+      const Tile tile = tiles[blockIdx.x];
+      const int tid = threadIdx.x;
+
+      // exchange 1: per thread, its first element's node x, v, director and the next node
+      __shared__ double s1[18][NT];
+      // exchange 2: per thread, its last element's c, P, R
+      __shared__ R s3[9][NT];
+
+      // ---- per element: rod, local index, slots, state -----------------------
+      int rodi[P];
+      int kk[P], nn[P];
+      bool valid[P];
+      uint64_t slot[P];
+      double x[P][3], v[P][3], q[P][9], w[P][3];
+      double tm[P], dt[P];
+      uint32_t rflags[P];
+      R gfj[P], gtj[P];
+      double xt[3] = {0, 0, 0}, vt[3] = {0, 0, 0};
+      int tailj = -1;
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        const uint32_t el = uint32_t(tid) * P + j;
+        valid[j] = el < tile.lanes;
+        // rod of this element: last entry with lane0 <= el
+        uint32_t lo = 0, hi = tile.count;
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (tile_lane0[tile.first + mid] <= el) lo = mid;
+          else hi = mid;
+        }
+        const uint32_t r = tile_rods[tile.first + lo];
+        rodi[j] = int(r);
+        const RodInfo& ri = rods[r];
+        kk[j] = valid[j] ? int(el - tile_lane0[tile.first + lo]) : 0;
+        nn[j] = int(ri.n);
+        slot[j] = ri.start + uint64_t(kk[j]);
+        dt[j] = ri.dt;
+        tm[j] = rod_time[r];
+        rflags[j] = ri.flags;
+        gfj[j] = static_cast<R>(ri.gamma_f);
+        gtj[j] = static_cast<R>(ri.gamma_t);
+        if (valid[j]) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            x[j][c] = X[c * N + slot[j]];
+            v[j][c] = Vel[c * N + slot[j]];
+            w[j][c] = W[c * E + slot[j]];
+          }
+#pragma unroll
+          for (int m = 0; m < 9; ++m) q[j][m] = Qd[m * E + slot[j]];
+          if (kk[j] == nn[j] - 1) {
+            tailj = j;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              xt[c] = X[c * N + slot[j] + 1];
+              vt[c] = Vel[c * N + slot[j] + 1];
+            }
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) x[j][c] = v[j][c] = w[j][c] = 0.0;
+#pragma unroll
+          for (int m = 0; m < 9; ++m) q[j][m] = (m % 4 == 0) ? 1.0 : 0.0;
+        }
+      }
+
+      for (int step = 0; step < nsteps; ++step) {
+        // ---- kinematic half step, time, post-kinematic clamp --------------------
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          if (!valid[j]) continue;
+          const double ch = 0.5 * dt[j];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) x[j][c] += ch * v[j][c];
+          rodrigues<R>(q[j], w[j], ch);
+          if (j == tailj)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) xt[c] += ch * vt[c];
+          tm[j] += 0.5 * dt[j];
+          if (kk[j] == 0 && (rflags[j] & 1u)) {
+            const RodInfo& ri = rods[rodi[j]];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) x[j][c] = ri.clamp_x[c];
+#pragma unroll
+            for (int m = 0; m < 9; ++m) q[j][m] = ri.clamp_q[m];
+          }
+        }
+        // ---- exchange 1 -----------------------------------------------------------
+        {
+          const bool last0 = valid[0] && (kk[0] == nn[0] - 1);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            s1[c][tid] = x[0][c];
+            s1[3 + c][tid] = v[0][c];
+            s1[15 + c][tid] = last0 ? xt[c] : (P > 1 ? x[P > 1 ? 1 : 0][c] : 0.0);
+          }
+#pragma unroll
+          for (int m = 0; m < 9; ++m) s1[6 + m][tid] = q[0][m];
+        }
+        __syncthreads();
+        Mid<R> mid[P];
+        const int tn = tid + 1 < NT ? tid + 1 : tid;
+        const int tn2 = tid + 2 < NT ? tid + 2 : tn;
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          if (!valid[j]) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+              mid[j].c[c] = mid[j].P[c] = mid[j].Rv[c] = mid[j].shear[c] = mid[j].lag[c] = mid[j].unst[c] = R(0);
+            mid[j].e = R(1);
+            continue;
+          }
+          const bool last = kk[j] == nn[j] - 1;
+          double xk1[3], xk2[3] = {0, 0, 0}, vk1[3], qk1[9];
+          if (last) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              xk1[c] = xt[c];
+              vk1[c] = vt[c];
+            }
+          } else if (j + 1 < P) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              xk1[c] = x[j + 1 < P ? j + 1 : j][c];
+              vk1[c] = v[j + 1 < P ? j + 1 : j][c];
+            }
+          } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              xk1[c] = s1[c][tn];
+              vk1[c] = s1[3 + c][tn];
+            }
+          }
+          const bool has_vor = !last;
+          if (has_vor) {
+            if (j + 1 < P) {
+#pragma unroll
+              for (int m = 0; m < 9; ++m) qk1[m] = q[j + 1 < P ? j + 1 : j][m];
+            } else {
+#pragma unroll
+              for (int m = 0; m < 9; ++m) qk1[m] = s1[6 + m][tn];
+            }
+            const bool next_last = kk[j] + 1 == nn[j] - 1;
+            if (next_last) {
+              if (j + 1 < P) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) xk2[c] = xt[c];
+              } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) xk2[c] = s1[15 + c][tn];
+              }
+            } else if (j + 2 < P) {
+#pragma unroll
+              for (int c = 0; c < 3; ++c) xk2[c] = x[j + 2 < P ? j + 2 : j][c];
+            } else if (j + 2 == P) {
+#pragma unroll
+              for (int c = 0; c < 3; ++c) xk2[c] = s1[c][tn];
+            } else if (P > 1) {
+#pragma unroll
+              for (int c = 0; c < 3; ++c) xk2[c] = s1[15 + c][tn];
+            } else {
+#pragma unroll
+              for (int c = 0; c < 3; ++c) xk2[c] = s1[c][tn2];
+            }
+          } else {
+#pragma unroll
+            for (int m = 0; m < 9; ++m) qk1[m] = 0.0;
+          }
+          element_forces<R>(x[j], xk1, xk2, v[j], vk1, q[j], qk1, w[j], has_vor, C, slot[j], E, mid[j]);
+        }
+        // ---- exchange 2 -----------------------------------------------------------
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          s3[c][tid] = mid[P - 1].c[c];
+          s3[3 + c][tid] = mid[P - 1].P[c];
+          s3[6 + c][tid] = mid[P - 1].Rv[c];
+        }
+        __syncthreads();
+        const int tp = tid > 0 ? tid - 1 : 0;
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          if (!valid[j]) continue;
+          const int k = kk[j];
+          const bool first = k == 0, last = k == nn[j] - 1;
+          R cm[3] = {R(0), R(0), R(0)}, Pm[3] = {R(0), R(0), R(0)}, Rm[3] = {R(0), R(0), R(0)};
+          if (!first) {
+            if (j > 0) {
+#pragma unroll
+              for (int c = 0; c < 3; ++c) {
+                cm[c] = mid[j > 0 ? j - 1 : 0].c[c];
+                Pm[c] = mid[j > 0 ? j - 1 : 0].P[c];
+                Rm[c] = mid[j > 0 ? j - 1 : 0].Rv[c];
+              }
+            } else {
+#pragma unroll
+              for (int c = 0; c < 3; ++c) {
+                cm[c] = s3[c][tp];
+                Pm[c] = s3[3 + c][tp];
+                Rm[c] = s3[6 + c][tp];
+              }
+            }
+          }
+          const uint64_t s = slot[j];
+          const Mid<R>& md = mid[j];
+          const R mk = C.m[s];
+          const R gf = gfj[j], gt = gtj[j];
+          // internal force on node k: difference of c, then damping
+          R f[3];
+          R vk[3] = {static_cast<R>(v[j][0]), static_cast<R>(v[j][1]), static_cast<R>(v[j][2])};
+          const R mg = mk * gf;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) f[c] = (first ? md.c[c] : md.c[c] - cm[c]) - mg * vk[c];
+          // internal torque on element k
+          const R em = R(0.5) * (mk + C.m[s + 1]);
+          const R eg = em * gt;
+          R tau[3];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            R d = first ? md.P[c] : (last ? -Pm[c] : md.P[c] - Pm[c]);
+            R qd = first ? R(0.5) * md.Rv[c] : (last ? R(0.5) * Rm[c] : R(0.5) * (md.Rv[c] + Rm[c]));
+            tau[c] = ((((d + qd) + md.shear[c]) + md.lag[c]) + md.unst[c]) - eg * static_cast<R>(w[j][c]);
+          }
+          // external loads of node k (forcing, then environment)
+          double fe[3] = {0.0, 0.0, 0.0};
+          node_environment(env, static_cast<double>(mk), x[j], v[j], fe);
+          const R im = C.invm[s];
+          const double dtt = dt[j];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const R a = (f[c] + static_cast<R>(fe[c])) * im;
+            v[j][c] += dtt * static_cast<double>(a);
+          }
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const R al = (C.iJ[c * E + s] * tau[c]) * md.e;
+            w[j][c] += dtt * static_cast<double>(al);
+          }
+          // last node of the rod
+          if (last) {
+            const R mn = C.m[s + 1];
+            const R mgn = mn * gf;
+            double fen[3] = {0.0, 0.0, 0.0};
+            if (rflags[j] & 2u) {
+              const RodInfo& ri = rods[rodi[j]];
+              const double rv = ramp_value(ri.ramp, tm[j], ri.ramp_start, ri.ramp_interval);
+#pragma unroll
+              for (int c = 0; c < 3; ++c) fen[c] = ri.force[c] * rv;
+            }
+            node_environment(env, static_cast<double>(mn), xt, vt, fen);
+            const R imn = C.invm[s + 1];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              const R fn = -md.c[c] - mgn * static_cast<R>(vt[c]);
+              const R a = (fn + static_cast<R>(fen[c])) * imn;
+              vt[c] += dtt * static_cast<double>(a);
+            }
+          }
+          // post-dynamic clamp
+          if (first && (rflags[j] & 1u)) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              v[j][c] = 0.0;
+              w[j][c] = 0.0;
+            }
+          }
+        }
+        // ---- time, kinematic half step ------------------------------------------
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          if (!valid[j]) continue;
+          tm[j] += 0.5 * dt[j];
+          const double ch = 0.5 * dt[j];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) x[j][c] += ch * v[j][c];
+          rodrigues<R>(q[j], w[j], ch);
+          if (j == tailj)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) xt[c] += ch * vt[c];
+        }
+      }
+
+      // ---- write back -------------------------------------------------------------
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        if (!valid[j]) continue;
+        const uint64_t s = slot[j];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          X[c * N + s] = x[j][c];
+          Vel[c * N + s] = v[j][c];
+          W[c * E + s] = w[j][c];
+        }
+#pragma unroll
+        for (int m = 0; m < 9; ++m) Qd[m * E + s] = q[j][m];
+        if (j == tailj) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            X[c * N + s + 1] = xt[c];
+            Vel[c * N + s + 1] = vt[c];
+          }
+        }
+        if (kk[j] == 0) rod_time[rodi[j]] = tm[j];
+      }
+    }
+
+
+    template <typename R, int P, int NT>
+    inline int launch_tiles(erod_batch* b, const Consts<R>& C, int steps) {
+      if (b->plan.n_tiles == 0) return EROD_OK;
+      hipLaunchKernelGGL((k_fused<R, P, NT>), dim3(static_cast<unsigned>(b->plan.n_tiles)), dim3(NT), 0, b->stream,
+                         b->d_field[EROD_POSITION], b->d_field[EROD_VELOCITY], b->d_field[EROD_DIRECTOR],
+                         b->d_field[EROD_ANGULAR_VELOCITY], C, b->d_rods, b->plan.d_tiles, b->plan.d_tile_rods,
+                         b->plan.d_tile_lane0, b->d_time, b->env, steps, b->E);
+      return hip_check(hipGetLastError(), "k_fused");
+    }
+
+    // one launch of `steps` steps over every tile, by tile capacity
+    template <typename R>
+    inline int launch_all(erod_batch* b, const Consts<R>& C, int steps) {
+      switch (b->plan.chosen_lanes) {
+        case 64: return launch_tiles<R, 1, 64>(b, C, steps);
+        case 128: return launch_tiles<R, 1, 128>(b, C, steps);
+        case 256: return launch_tiles<R, 1, 256>(b, C, steps);
+        case 512: return launch_tiles<R, 1, 512>(b, C, steps);
+        case 1024: return launch_tiles<R, 2, 512>(b, C, steps);
+        default: set_error("bad tile size"); return EROD_ERR_INVALID;
+      }
+    }
+
+  }  // namespace fused
+
+  int fused_launch_fp64(erod_batch* b, const fused::Consts<double>& C, int steps);
+  int fused_launch_fp32(erod_batch* b, const fused::Consts<float>& C, int steps);
+
+}  // namespace erod

@@ -1,0 +1,145 @@
+"""Seeded synthetic rod workloads (host side, FP64).
+
+Each builder returns a Workload: per-rod element counts, per-rod state dicts
+(tag -> (ncomp, count) arrays, ready for `assemble`), a timestep, and hook
+descriptors.  The same arrays feed the GPU paths and the CPU oracle, so the
+random generator does not affect parity.
+
+  timoshenko_c0      1 rod x 100 elements, clamped, tip force (0, -15, 0)
+  uniform_c1         1024 rods x 64 elements, random rest curvature and velocity
+  ragged_c2          4096 rods, n log-uniform in [16, 1024], random materials,
+                     constant tip forces
+  contact_c4         rods lying above the plane z = 0 under gravity (contact)
+
+Material conventions: G = E / (1 + 0.99) (a Poisson ratio of 0.99 as stated
+for the Timoshenko configuration); damping rates for forces and torques are
+equal.  The reference's timestep rule dt = 0.01 L / n
+(examples/timoshenko/TimoshenkoCaseStudy.hpp:136-143) is tuned for the axial
+wave speed sqrt(E / rho) = sqrt(1e6 / 5e3); for other materials the same
+Courant number is kept: dt_i = 0.01 (L_i / n_i) sqrt(1e6 / 5e3) / sqrt(E_i / rho_i),
+and a batch uses the smallest dt of its rods (the reference steps every
+system with one dt).
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import DeviceBlock, assemble, straight_rod_state
+
+C_REF = np.sqrt(1e6 / 5e3)
+
+
+@dataclass
+class Workload:
+    name: str
+    n_elems: list
+    states: list
+    dt: float
+    steps: int
+    clamps: list = field(default_factory=list)   # (rod, x[3], Q[9])
+    forces: list = field(default_factory=list)   # (rod, F[3], ramp, start, interval)
+    gravity: tuple = None
+    contact: tuple = None                        # (k, c, mu, ct, z0, offset)
+    info: dict = field(default_factory=dict)
+
+    @property
+    def total_elements(self):
+        return int(sum(self.n_elems))
+
+
+def stable_dt(length, n, youngs, density):
+    return 0.01 * (length / n) * C_REF / np.sqrt(youngs / density)
+
+
+def timoshenko_c0(steps=50000):
+    n, L, r, rho, E = 100, 3.0, 0.25, 5000.0, 1e6
+    G = E / (1.0 + 0.99)
+    st = straight_rod_state(n, r, L, [0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0], rho, E, G, 0.1, 0.1)
+    dt = 0.01 * (L / n)
+    clamps = [(0, st["Position"][:, 0].copy(), st["Director"][:, 0].copy())]
+    forces = [(0, np.array([0.0, -15.0, 0.0]), 0, 0.0, 1.0)]
+    return Workload("timoshenko_c0", [n], [st], dt, steps, clamps=clamps, forces=forces,
+                    info=dict(length=L, radius=r, density=rho, youngs=E, shear=G, damping=0.1))
+
+
+def _randomize(st, rng, kappa_std=0.5, v_std=1e-2):
+    n = st["Director"].shape[1]
+    st["ReferenceCurvature"] = rng.normal(0.0, kappa_std, (3, n - 1))
+    st["Velocity"] = rng.normal(0.0, v_std, (3, n + 1))
+    return st
+
+
+def uniform_c1(n_rods=1024, n=64, steps=10000, seed=1001):
+    rng = np.random.default_rng(seed)
+    L, r, rho, E = 3.0, 0.25, 5000.0, 1e6
+    G = E / (1.0 + 0.99)
+    states = []
+    for i in range(n_rods):
+        st = straight_rod_state(n, r, L, [0.0, 1.0 * i, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0], rho, E, G, 0.1, 0.1)
+        states.append(_randomize(st, rng))
+    dt = 0.01 * (L / n)
+    return Workload("uniform_c1", [n] * n_rods, states, dt, steps,
+                    info=dict(length=L, radius=r, density=rho, youngs=E, shear=G, damping=0.1))
+
+
+def ragged_c2(n_rods=4096, steps=10000, seed=2002, n_min=16, n_max=1024):
+    rng = np.random.default_rng(seed)
+    n_elems = np.rint(np.exp(rng.uniform(np.log(n_min), np.log(n_max), n_rods))).astype(int)
+    n_elems = np.clip(n_elems, n_min, n_max)
+    L = rng.uniform(0.5, 3.0, n_rods)
+    r = rng.uniform(0.005, 0.05, n_rods)
+    E = np.exp(rng.uniform(np.log(1e5), np.log(1e7), n_rods))
+    rho = rng.uniform(500.0, 5000.0, n_rods)
+    gam = rng.uniform(0.01, 0.5, n_rods)
+    F = rng.normal(0.0, 1.0, (n_rods, 3))
+    states, forces = [], []
+    for i in range(n_rods):
+        st = straight_rod_state(int(n_elems[i]), r[i], L[i], [0.0, 0.2 * i, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0],
+                                rho[i], E[i], E[i] / (1.0 + 0.99), gam[i], gam[i])
+        states.append(_randomize(st, rng))
+        forces.append((i, F[i].copy(), 0, 0.0, 1.0))
+    dts = stable_dt(L, n_elems, E, rho)
+    return Workload("ragged_c2", [int(x) for x in n_elems], states, float(dts.min()), steps, forces=forces,
+                    info=dict(rod_dt_min=float(dts.min()), rod_dt_max=float(dts.max())))
+
+
+def contact_c4(n_rods=100000, n=128, steps=2000, seed=4004, k=1e3, c=10.0, mu=0.3, ct=1e2):
+    """Rods of length 1 m and radius 0.01 m lying on z = radius (just touching
+    the plane z = 0 with the contact offset equal to the radius), random rest
+    curvature, gravity (0, 0, -9.81), per-node plane contact k = 1e3, c = 10,
+    mu = 0.3 and a tangential damping ct = 100 for the regularised friction."""
+    rng = np.random.default_rng(seed)
+    L, r, rho, E = 1.0, 0.01, 1000.0, 1e6
+    G = E / (1.0 + 0.99)
+    base = straight_rod_state(n, r, L, [0.0, 0.0, r], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0], rho, E, G, 0.1, 0.1)
+    states = []
+    for i in range(n_rods):
+        st = {k_: v.copy() for k_, v in base.items()}
+        st["Position"] = base["Position"].copy()
+        st["Position"][1] += 0.05 * i
+        st["ReferenceCurvature"] = rng.normal(0.0, 0.5, (3, n - 1))
+        states.append(st)
+    dt = stable_dt(L, n, E, rho)
+    return Workload("contact_c4", [n] * n_rods, states, float(dt), steps, gravity=(0.0, 0.0, -9.81),
+                    contact=(k, c, mu, ct, 0.0, r), info=dict(length=L, radius=r, density=rho, youngs=E))
+
+
+def apply_hooks(target, wl):
+    """Register the workload's hooks on a DeviceBlock or an oracle System."""
+    for rod, x, Q in wl.clamps:
+        target.add_clamp(rod, x, Q)
+    for rod, F, ramp, start, interval in wl.forces:
+        target.add_end_force(rod, F, ramp, start, interval)
+    if wl.gravity is not None:
+        target.set_gravity(wl.gravity)
+    if wl.contact is not None:
+        target.set_plane_contact(*wl.contact)
+
+
+def build_block(wl, damped=True):
+    """A DeviceBlock holding the workload's rods, hooks and (uniform) dt."""
+    blk = DeviceBlock(wl.n_elems, damped=damped)
+    assemble(blk, wl.states)
+    apply_hooks(blk, wl)
+    blk.set_rod_dt(wl.dt)
+    return blk

@@ -1,0 +1,172 @@
+/*
+ * erod.h — C ABI of the MI355X-native Cosserat-rod block timestep.
+ *
+ * A batch is one device-resident, ghost-padded multi-rod block: the
+ * reference's blocks::Block<CosseratRod> (elastica/Systems/CosseratRods/Block.hpp)
+ * with every per-variable array moved into HBM.  Rod i starts at the same
+ * slot s_i in the node, element and Voronoi arrays, s_{i+1} = s_i + n_i + 2
+ * (Block.hpp:401-443); each field is component-major (component c of slot k
+ * at c * slots + k; a director is 9 lines, line 3p + r holding Q(p, r)).
+ *
+ * All calls return EROD_OK (0) or a negative erod_status.  A batch owns one
+ * HIP stream; calls on one batch are ordered on that stream and are not
+ * thread-safe.  Host pointers are plain host memory.
+ */
+#ifndef EROD_H
+#define EROD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  EROD_OK = 0,
+  EROD_ERR_INVALID = -1,      /* bad argument (size, index, field, n_elem < 2) */
+  EROD_ERR_HIP = -2,          /* a HIP runtime call failed; see erod_last_error */
+  EROD_ERR_UNSUPPORTED = -3,  /* the requested path cannot run this batch */
+  EROD_ERR_NO_DEVICE = -4,    /* no gfx950 device visible */
+  EROD_ERR_NONFINITE = -5     /* a non-finite state value was detected */
+} erod_status;
+
+/* Field ids: the reference's tags (elastica/Systems/CosseratRods/Tags.hpp and
+ * the component variable lists).  Rigidity, inertia and inverse inertia are
+ * stored as their diagonals (3 lines): the reference's 3x3 tensors are
+ * diagonal and its matvec with them only adds exact zeros
+ * (WithDiagonalLinearHyperElasticModel.hpp:358, WithRodKinematics.hpp:358-364). */
+typedef enum {
+  /* nodes (n + 1 per rod) */
+  EROD_POSITION = 0, EROD_VELOCITY, EROD_ACCELERATION, EROD_MASS, EROD_INV_MASS,
+  EROD_INTERNAL_LOADS, EROD_EXTERNAL_LOADS, EROD_FORCE_DAMPING_RATE,
+  /* elements (n per rod) */
+  EROD_DIRECTOR, EROD_ANGULAR_VELOCITY, EROD_ANGULAR_ACCELERATION,
+  EROD_REFERENCE_ELEMENT_LENGTH, EROD_ELEMENT_LENGTH, EROD_ELEMENT_DILATATION,
+  EROD_TANGENT, EROD_ELEMENT_DIMENSION, EROD_ELEMENT_VOLUME,
+  EROD_SHEAR_STRETCH_RIGIDITY, EROD_MASS_SECOND_MOMENT_OF_INERTIA,
+  EROD_INV_MASS_SECOND_MOMENT_OF_INERTIA, EROD_SHEAR_STRETCH_STRAIN,
+  EROD_REFERENCE_SHEAR_STRETCH_STRAIN, EROD_INTERNAL_STRESS, EROD_INTERNAL_TORQUES,
+  EROD_EXTERNAL_TORQUES, EROD_TORQUE_DAMPING_RATE, EROD_DUMMY_ELEMENT_VECTOR,
+  EROD_DUMMY_ELEMENT_VECTOR2,
+  /* voronoi (n - 1 per rod) */
+  EROD_REFERENCE_VORONOI_LENGTH, EROD_VORONOI_LENGTH, EROD_VORONOI_DILATATION,
+  EROD_CURVATURE, EROD_REFERENCE_CURVATURE, EROD_BENDING_TWIST_RIGIDITY,
+  EROD_INTERNAL_COUPLE, EROD_DUMMY_VORONOI_VECTOR,
+  EROD_N_FIELDS
+} erod_field;
+
+typedef enum { EROD_NODE = 0, EROD_ELEMENT = 1, EROD_VORONOI = 2 } erod_stagger;
+
+typedef enum {
+  EROD_FP64 = 0,       /* everything in FP64 */
+  EROD_FP32_BULK = 1   /* FP64 state, integrator and exponential map; FP32 strains/stresses/loads */
+} erod_precision;
+
+typedef enum { EROD_RAMP_NONE = 0, EROD_RAMP_LINEAR = 1, EROD_RAMP_SIN = 2 } erod_ramp;
+
+typedef struct erod_batch erod_batch;
+
+/* ---- lifetime and layout ------------------------------------------------ */
+
+/* Replaces Block::emplace for n_rods rods at once (Block.hpp:568-611): sizes
+ * the ghost-padded arrays, fills every ghost slot (Block.hpp:815-849) and
+ * zeroes the rest.  `damped` selects CosseratRod (1) or
+ * CosseratRodWithoutDamping (0) (Aliases/CosseratRod*.hpp). */
+int erod_batch_create(const uint64_t* n_elems, uint64_t n_rods, int damped, erod_batch** out);
+int erod_batch_destroy(erod_batch* b);
+/* Device ordinal the batch lives on. */
+int erod_batch_device(const erod_batch* b, int* device);
+/* Total element slots (ghosts included); nodes = slots + 1, voronoi = slots - 1. */
+int erod_batch_slots(const erod_batch* b, uint64_t* element_slots);
+/* First slot of a rod (Block::get_start_of_slice, Block.hpp:216-227). */
+int erod_rod_start(const erod_batch* b, uint64_t rod, uint64_t* start);
+/* Lines and stagger of a field. */
+int erod_field_shape(int field, int* ncomp, int* stagger);
+
+/* ---- data movement ------------------------------------------------------- */
+
+/* Whole-block copies: host holds ncomp lines of the stagger's slot count. */
+int erod_upload_field(erod_batch* b, int field, const double* host);
+int erod_download_field(erod_batch* b, int field, double* host);
+/* One rod's slice (blocks::get<tag>(slice)): host holds ncomp lines of the
+ * rod's n + 1 / n / n - 1 values. */
+int erod_upload_rod(erod_batch* b, int field, uint64_t rod, const double* host);
+int erod_download_rod(erod_batch* b, int field, uint64_t rod, double* host);
+/* Rewrite every ghost slot (fill_ghosts, Block.hpp:815-849). */
+int erod_fill_ghosts(erod_batch* b);
+
+/* ---- the three per-block calls of the stepper (op-by-op FP64 path) ------- */
+
+/* PositionVerlet::KinematicSubStep (PositionVerlet.hpp:60-69): x += (dt/2) v,
+ * Q <- exp(-(dt/2)[w]x) Q over every slot. */
+int erod_kinematic_update(erod_batch* b, double dt);
+/* ExternalLoads = ExternalTorques = 0, then compute_internal_dynamics
+ * (SymplecticSteppers.hpp:178-188, ElasticityInterface.hpp:360-365). */
+int erod_compute_internal_dynamics(erod_batch* b, double time);
+/* PositionVerlet::DynamicSubStep with update_acceleration
+ * (PositionVerlet.hpp:71-77, WithRodKinematics.hpp:683-714). */
+int erod_dynamic_update(erod_batch* b, double dt);
+
+/* Device-side hook descriptors that replace per-step host lambdas.
+ * Clamp: value constraint after the kinematic half step (position of node
+ * `index`, director of element `index`), rate constraint after the dynamic
+ * step (ConstrainPrimitives / ConstrainRates,
+ * elastica/Simulator/Interface/Constraints/ExecutionPolicy.hpp:591-593).
+ * Only index 0 is supported by the fused path. */
+int erod_add_clamp(erod_batch* b, uint64_t rod, uint64_t index, const double x[3], const double Q[9]);
+/* ExternalLoads(:, last node) = F * ramp(t) before the dynamic step
+ * (examples/timoshenko/TimoshenkoCaseStudy.cpp:128-141; ramps
+ * elastica/Utilities/Rampup.hpp:95-150). */
+int erod_add_end_force(erod_batch* b, uint64_t rod, const double F[3], int ramp, double start, double interval);
+/* ExternalLoads += g (x) Mass (GravityField.hpp:239-246). */
+int erod_set_gravity(erod_batch* b, const double g[3]);
+/* Per-node plane contact z = z0 + offset with a linear spring-dashpot normal
+ * force max(0, k delta + c v_n) and friction min(ct |v_t|, mu f_N) against
+ * the slip (LinearSpringDashpot.cpp:49-71 applied per node). */
+int erod_set_plane_contact(erod_batch* b, double k, double c, double mu, double ct, double z0, double offset);
+int erod_apply_post_kinematic(erod_batch* b, double time);
+int erod_apply_pre_dynamic(erod_batch* b, double time);
+int erod_apply_post_dynamic(erod_batch* b, double time);
+
+/* n full position-Verlet steps through the op-by-op kernels
+ * (SymplecticSteppers.hpp:133-217); *time advances by n dt. */
+int erod_step(erod_batch* b, double* time, double dt, uint64_t n);
+
+/* ---- fused / persistent path ---------------------------------------------- */
+
+/* Per-rod timestep for the fused path (rods are independent; default: 0 = unset). */
+int erod_set_rod_dt(erod_batch* b, const double* dt_per_rod);
+/* Per-rod clock (seconds), read back after fused steps. */
+int erod_get_rod_time(erod_batch* b, double* time_per_rod);
+int erod_set_rod_time(erod_batch* b, const double* time_per_rod);
+/* Tile / launch shape of the persistent kernels.  lanes_per_tile in
+ * {64,128,256,512,1024}; 0 lets the autotuner decide.  steps_per_launch: steps a
+ * workgroup advances with its rods' state held on chip. */
+int erod_set_fused_config(erod_batch* b, int lanes_per_tile, int steps_per_launch);
+int erod_get_fused_config(const erod_batch* b, int* lanes_per_tile, int* steps_per_launch);
+/* n position-Verlet steps with the fused persistent kernels at the given
+ * precision.  Keeps x, v, Q, w (and the rod clocks) current; derived fields
+ * are refreshed on request by erod_compute_internal_dynamics. */
+int erod_step_fused(erod_batch* b, int precision, uint64_t n);
+/* Same, replaying a captured hipGraph of `chunk` steps n / chunk times
+ * (n must be a multiple of chunk). */
+int erod_step_fused_graph(erod_batch* b, int precision, uint64_t n, uint64_t chunk);
+/* Re-orthonormalise every director (one Newton-Schulz polar step, FP64). */
+int erod_reorthonormalize(erod_batch* b);
+/* Average duration (ms) of the dominant kernel over the last fused call,
+ * measured with HIP events on the batch stream. */
+int erod_last_kernel_ms(const erod_batch* b, double* ms, uint64_t* launches);
+
+/* ---- synchronisation and errors ------------------------------------------ */
+int erod_synchronize(erod_batch* b);
+/* Flags a non-finite value in x, v, Q or w of any rod (0/1 per rod). */
+int erod_check_finite(erod_batch* b, uint8_t* rod_flags);
+/* Message of the last failed call in this thread. */
+const char* erod_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EROD_H */

@@ -635,9 +635,10 @@ namespace oracle {
     }
   }
 
+  // Module order Forcing, then Environments (make_simulator<Forcing, ...,
+  // Environments>; Simulator.hpp:284-285 appends each module's steps in order).
   template <typename T>
   void apply_pre_dynamic(System<T>& s, double t) {
-    for (auto& b : s.blocks) apply_environment(b, s);
     for (auto const& f : s.forces) {
       auto [bi, ri] = s.where[f.rod];
       auto& b = s.blocks[bi];
@@ -646,6 +647,7 @@ namespace oracle {
       const double r = ramp_value(f.ramp, t, f.start, f.interval);
       for (int d = 0; d < 3; ++d) b.get(ExternalLoads)[d * N + node] = T(f.F[d] * r);
     }
+    for (auto& b : s.blocks) apply_environment(b, s);
   }
 
   // One position-Verlet step over all blocks

@@ -149,7 +149,7 @@ class System:
     def kinematic_update(self, dt):
         _lib().oracle_kinematic_update(self._h, dt)
 
-    def add_clamp(self, rod, index, x, Q):
+    def add_clamp(self, rod, x, Q, index=0):
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(3)
         Q = np.ascontiguousarray(Q, dtype=np.float64).reshape(9)
         _lib().oracle_add_clamp(self._h, rod, index, _dptr(x), _dptr(Q))

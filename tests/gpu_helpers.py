@@ -1,0 +1,63 @@
+"""Shared builders for the GPU tests: the same per-rod arrays go into a
+DeviceBlock (product) and an oracle System (checker)."""
+import numpy as np
+
+from elasticapp_amd import straight_rod_state
+from elasticapp_amd.workloads import Workload, apply_hooks, build_block
+from oracle import pyoracle
+
+STATE = ("Position", "Velocity", "Director", "AngularVelocity")
+
+
+def random_workload(n_rods, n_min, n_max, seed, steps=100, forces=True, kappa=0.5, vstd=1e-2):
+    rng = np.random.default_rng(seed)
+    n_elems = rng.integers(n_min, n_max + 1, n_rods)
+    states, fl = [], []
+    dts = []
+    for i in range(n_rods):
+        n = int(n_elems[i])
+        L = rng.uniform(0.3, 2.0)
+        r = rng.uniform(0.005, 0.04)
+        E = np.exp(rng.uniform(np.log(1e5), np.log(1e7)))
+        rho = rng.uniform(500.0, 5000.0)
+        g = rng.uniform(0.01, 0.5)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        nrm = np.cross(d, rng.normal(size=3))
+        nrm /= np.linalg.norm(nrm)
+        st = straight_rod_state(n, r, L, rng.normal(size=3), d, nrm, rho, E, E / 1.5, g, g)
+        st["ReferenceCurvature"] = rng.normal(0.0, kappa, (3, n - 1))
+        st["ReferenceShearStretchStrain"] = rng.normal(0.0, 1e-3, (3, n))
+        st["Velocity"] = rng.normal(0.0, vstd, (3, n + 1))
+        st["AngularVelocity"] = rng.normal(0.0, 10 * vstd, (3, n))
+        states.append(st)
+        if forces:
+            fl.append((i, rng.normal(0.0, 0.1, 3), 0, 0.0, 1.0))
+        dts.append(0.01 * (L / n) * np.sqrt(1e6 / 5e3) / np.sqrt(E / rho))
+    return Workload("random%d" % seed, [int(x) for x in n_elems], states, float(min(dts)), steps, forces=fl)
+
+
+def device_block(wl, damped=True):
+    return build_block(wl, damped=damped)
+
+
+def oracle_system(wl, damped=True, max_block_elems=0):
+    s = pyoracle.System(wl.n_elems, max_block_elems=max_block_elems, damped=damped)
+    for r, st in enumerate(wl.states):
+        for tag, a in st.items():
+            s.rod_field(r, tag)[...] = a
+    apply_hooks(s, wl)
+    return s
+
+
+def rel_state_error(blk, sysm, tags=STATE, rods=None):
+    """max over rods and tags of |gpu - cpu| / max|cpu| (per tag and rod)."""
+    worst = {}
+    for r in (range(len(blk.n_elems)) if rods is None else rods):
+        for t in tags:
+            g = blk.get_rod(t, r)
+            c = sysm.rod_field(r, t)
+            scale = max(np.abs(c).max(), 1e-300)
+            e = np.abs(g - c).max() / scale
+            worst[t] = max(worst.get(t, 0.0), e)
+    return worst

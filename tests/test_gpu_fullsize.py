@@ -1,0 +1,69 @@
+"""Full-size ragged batch (4096 rods, n log-uniform in [16, 1024]) on the GPU:
+properties that do not need the CPU oracle at this size.
+
+  * every path stays finite;
+  * the fused FP64 path equals the op-by-op FP64 path (itself pinned to the
+    oracle and the golden vectors) to <= 1e-10 relative;
+  * the FP32-bulk path stays within 1e-6 of FP64 in position;
+  * a sample of rods stepped alone reproduces their in-batch state bitwise.
+"""
+import numpy as np
+import pytest
+
+from elasticapp_amd import FP32_BULK, FP64
+from elasticapp_amd.workloads import Workload, build_block, ragged_c2
+from tests.gpu_helpers import STATE
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 200
+
+
+@pytest.fixture(scope="module")
+def c2():
+    return ragged_c2(steps=STEPS)
+
+
+@pytest.fixture(scope="module")
+def fused64(c2):
+    blk = build_block(c2)
+    blk.step_fused(STEPS, FP64)
+    yield blk
+    blk.close()
+
+
+def test_c2_layout(c2):
+    n = np.array(c2.n_elems)
+    assert len(n) == 4096 and n.min() >= 16 and n.max() <= 1024
+    assert 0.9e6 < n.sum() < 1.1e6
+
+
+def test_c2_fused_fp64_equals_op_by_op(c2, fused64):
+    assert not fused64.check_finite().any()
+    ops = build_block(c2)
+    ops.step(c2.dt, STEPS)
+    for t in STATE:
+        a, b = fused64.get_field(t), ops.get_field(t)
+        err = np.abs(a - b).max() / np.abs(b).max()
+        assert err <= 1e-10, (t, err)
+    ops.close()
+
+
+def test_c2_fp32_bulk_close_to_fp64(c2, fused64):
+    blk = build_block(c2)
+    blk.step_fused_graph(STEPS, 100, FP32_BULK)
+    assert not blk.check_finite().any()
+    x32, x64 = blk.get_field("Position"), fused64.get_field("Position")
+    err = np.abs(x32 - x64).max() / np.abs(x64).max()
+    assert err <= 1e-6, err
+    blk.close()
+
+
+def test_c2_rods_alone_equal_in_batch(c2, fused64):
+    for r in (0, 1234, 4095, int(np.argmax(c2.n_elems))):
+        one = Workload("one", [c2.n_elems[r]], [c2.states[r]], c2.dt, STEPS, forces=[(0,) + tuple(c2.forces[r][1:])])
+        blk = build_block(one)
+        blk.step_fused(STEPS, FP64)
+        for t in STATE:
+            assert np.array_equal(blk.get_rod(t, 0), fused64.get_rod(t, r)), (r, t)
+        blk.close()

@@ -1,0 +1,195 @@
+"""GPU parity: every path of liberod.so against the reference's golden vectors
+and the CPU oracle.
+
+Tolerances
+  * op-by-op FP64 path: the reference's golden tolerances (4e-13 .. 4e-12);
+    against the oracle <= 1e-12 relative (same operation order, no FMA; only
+    the device acos/sin/cos differ from glibc by an ulp).
+  * fused FP64 path: <= 1e-10 relative after hundreds of steps (FMA
+    contraction and the summation order of the torque terms differ).
+  * FP32-bulk path: position error <= 1e-6 relative to the position scale.
+"""
+import numpy as np
+import pytest
+
+from elasticapp_amd import FP32_BULK, FP64, DeviceBlock, straight_rod_state
+from tests.golden_fixtures import as_field, load, rel_err_ok
+from tests.gpu_helpers import STATE, device_block, oracle_system, random_workload, rel_state_error
+
+pytestmark = pytest.mark.gpu
+
+
+# --- golden vectors ---------------------------------------------------------------
+
+def _golden_block(case):
+    p = case["init"]
+    n = p["n_elem"]
+    blk = DeviceBlock([n] * case["n_rods"], damped=case["plugin"] == "CosseratRod")
+    st = straight_rod_state(n, p["radius"], p["length"], p["origin"], p["direction"], p["normal"], p["density"],
+                            p["youngs_modulus"], p["shear_modulus"], p["force_damping"] or 0.0,
+                            p["torque_damping"] or 0.0)
+    for r in range(case["n_rods"]):
+        for tag, a in st.items():
+            blk.set_rod(tag, r, a)
+    for a in case["assign"]:
+        entry = case["replaced"][str(a["rod"])][a["tag"]]
+        blk.set_rod(a["tag"], a["rod"], as_field(entry, a["tag"], a["shrink_diagonal"]))
+    return blk
+
+
+def _golden_cases():
+    out = []
+    for f in ("fullrod_PerturbedRodStep.json", "fullrod_PerturbedRodStepWithoutDamping.json"):
+        for case in load(f)["cases"]:
+            for sec in case["sections"]:
+                out.append(pytest.param(case, sec, id="%s-%s" % (case["name"].split(".")[-2], sec["name"])))
+    return out
+
+
+@pytest.mark.parametrize("case,section", _golden_cases())
+def test_golden_op_by_op(case, section):
+    blk = _golden_block(case)
+    blk.step(case["dt"], section["steps"])
+    bad = []
+    for rod, vars_ in section["expected"].items():
+        for tag, entry in vars_.items():
+            ok = rel_err_ok(blk.get_rod(tag, int(rod)), as_field(entry, tag), section["epsilon"])
+            if not ok.all():
+                bad.append("rod%s %s %d/%d" % (rod, tag, (~ok).sum(), ok.size))
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("case,section", [p for p in _golden_cases() if p.values[1]["steps"] > 0])
+@pytest.mark.parametrize("precision", [FP64])
+def test_golden_fused_state(case, section, precision):
+    blk = _golden_block(case)
+    blk.set_rod_dt(case["dt"])
+    blk.step_fused(section["steps"], precision)
+    bad = []
+    for rod, vars_ in section["expected"].items():
+        for tag in STATE:
+            ok = rel_err_ok(blk.get_rod(tag, int(rod)), as_field(vars_[tag], tag), max(section["epsilon"], 1e-12))
+            if not ok.all():
+                bad.append("rod%s %s %d/%d" % (rod, tag, (~ok).sum(), ok.size))
+    assert not bad, bad
+
+
+# --- the oracle on random ragged batches ----------------------------------------------
+
+@pytest.fixture(scope="module")
+def ragged():
+    wl = random_workload(40, 2, 300, seed=7, steps=300)
+    sysm = oracle_system(wl)
+    sysm.step(wl.dt, wl.steps)
+    return wl, sysm
+
+
+def test_op_by_op_matches_oracle(ragged):
+    wl, sysm = ragged
+    blk = device_block(wl)
+    blk.step(wl.dt, wl.steps)
+    err = rel_state_error(blk, sysm, STATE + ("InternalLoads", "InternalTorques", "Curvature", "ShearStretchStrain"))
+    assert max(err.values()) <= 1e-12, err
+
+
+def test_fused_fp64_matches_oracle(ragged):
+    wl, sysm = ragged
+    blk = device_block(wl)
+    blk.step_fused(wl.steps, FP64)
+    err = rel_state_error(blk, sysm)
+    assert max(err.values()) <= 1e-10, err
+    assert np.allclose(blk.rod_time(), sysm.time, rtol=1e-14)
+
+
+def test_fused_fp32_bulk_error_vs_oracle(ragged):
+    wl, sysm = ragged
+    blk = device_block(wl)
+    blk.step_fused(wl.steps, FP32_BULK)
+    err = rel_state_error(blk, sysm)
+    assert err["Position"] <= 1e-6, err
+    assert err["Director"] <= 1e-5, err
+
+
+@pytest.mark.parametrize("lanes", [256, 512, 1024])
+def test_fused_is_independent_of_tiling(lanes):
+    """Per-rod arithmetic does not depend on which tile or lane a rod lands in."""
+    wl = random_workload(30, 2, 250, seed=11, steps=50)
+    ref = device_block(wl)
+    ref.set_fused_config(1024, 25)
+    ref.step_fused(wl.steps, FP64)
+    blk = device_block(wl)
+    blk.set_fused_config(lanes, 10)
+    blk.step_fused(wl.steps, FP64)
+    for r in range(len(wl.n_elems)):
+        for t in STATE:
+            assert np.array_equal(blk.get_rod(t, r), ref.get_rod(t, r)), (lanes, r, t)
+
+
+def test_fused_graph_replay_equals_direct():
+    wl = random_workload(20, 2, 120, seed=13, steps=60)
+    a = device_block(wl)
+    a.set_fused_config(0, 20)
+    a.step_fused(60, FP32_BULK)
+    b = device_block(wl)
+    b.set_fused_config(0, 20)
+    b.step_fused_graph(60, 20, FP32_BULK)
+    for r in range(len(wl.n_elems)):
+        for t in STATE:
+            assert np.array_equal(a.get_rod(t, r), b.get_rod(t, r)), (r, t)
+
+
+def test_rod_alone_equals_rod_in_batch():
+    wl = random_workload(12, 2, 90, seed=17, steps=40)
+    full = device_block(wl)
+    full.step_fused(wl.steps, FP64)
+    from elasticapp_amd.workloads import Workload
+    for r in (0, 5, 11):
+        one = Workload("one", [wl.n_elems[r]], [wl.states[r]], wl.dt, wl.steps,
+                       forces=[(0,) + tuple(wl.forces[r][1:])])
+        blk = device_block(one)
+        blk.step_fused(wl.steps, FP64)
+        for t in STATE:
+            assert np.array_equal(blk.get_rod(t, 0), full.get_rod(t, r)), (r, t)
+
+
+# --- hooks: clamp, ramped tip force, gravity, plane contact ---------------------------
+
+def _hooked_workload():
+    wl = random_workload(6, 8, 60, seed=23, steps=400, forces=False)
+    for r in range(3):
+        st = wl.states[r]
+        wl.clamps.append((r, st["Position"][:, 0].copy(), st["Director"][:, 0].copy()))
+        wl.forces.append((r, np.array([0.3, -0.2, 0.1]), r % 3, 0.0, wl.dt * 200))
+    wl.gravity = (0.0, 0.0, -9.81)
+    zmin = min(st["Position"][2].min() for st in wl.states)
+    wl.contact = (1e3, 10.0, 0.3, 100.0, zmin + 0.01, 0.0)
+    return wl
+
+
+def test_hooks_op_by_op_and_fused_match_oracle():
+    wl = _hooked_workload()
+    sysm = oracle_system(wl)
+    sysm.step(wl.dt, wl.steps)
+    a = device_block(wl)
+    a.step(wl.dt, wl.steps)
+    err = rel_state_error(a, sysm)
+    assert max(err.values()) <= 1e-12, err
+    b = device_block(wl)
+    b.step_fused(wl.steps, FP64)
+    err = rel_state_error(b, sysm)
+    assert max(err.values()) <= 1e-10, err
+    # clamped ends stay put
+    for r in range(3):
+        assert np.array_equal(b.get_rod("Position", r)[:, 0], wl.states[r]["Position"][:, 0])
+
+
+def test_finite_flags():
+    wl = random_workload(5, 4, 20, seed=29, steps=10)
+    blk = device_block(wl)
+    blk.step_fused(10, FP64)
+    assert not blk.check_finite().any()
+    v = blk.get_rod("Velocity", 3)
+    v[1, 2] = np.nan
+    blk.set_rod("Velocity", 3, v)
+    flags = blk.check_finite()
+    assert flags.tolist() == [0, 0, 0, 1, 0]

@@ -165,7 +165,7 @@ def test_axial_stretching_tip_displacement():
     n, L, r, E, F = 10, 1.0, 0.025, 1e4, 1.0
     s = pyoracle.System([n])
     s.init_straight_rod(0, r, L, [0, 0, 0], [1, 0, 0], [0, 1, 0], 1e3, E, E / 1.5, 10.0, 10.0)
-    s.add_clamp(0, 0, s.rod_field(0, "Position")[:, 0].copy(), s.rod_field(0, "Director")[:, 0].copy())
+    s.add_clamp(0, s.rod_field(0, "Position")[:, 0].copy(), s.rod_field(0, "Director")[:, 0].copy())
     s.add_end_force(0, [F, 0, 0], ramp=1, start=0.0, interval=2.0)
     dt = 0.01 * L / n
     s.step(dt, int(20.0 / dt))
@@ -179,7 +179,7 @@ def _timoshenko_error(n):
     L, r, rho, E, G, F = 3.0, 0.25, 5e3, 1e6, 1e4, -15.0
     s = pyoracle.System([n])
     s.init_straight_rod(0, r, L, [0, 0, 0], [0, 0, 1], [0, 1, 0], rho, E, G, 0.5, 0.5)
-    s.add_clamp(0, 0, s.rod_field(0, "Position")[:, 0].copy(), s.rod_field(0, "Director")[:, 0].copy())
+    s.add_clamp(0, s.rod_field(0, "Position")[:, 0].copy(), s.rod_field(0, "Director")[:, 0].copy())
     s.add_end_force(0, [F, 0, 0], ramp=2, start=0.0, interval=100.0)
     dt = 0.01 * L / n
     s.step(dt, int(200.0 / dt))
