@@ -89,8 +89,9 @@ def load_library():
         "erod_set_rod_dt": (i32, [vp, P(dbl)]),
         "erod_get_rod_time": (i32, [vp, P(dbl)]),
         "erod_set_rod_time": (i32, [vp, P(dbl)]),
-        "erod_set_fused_config": (i32, [vp, i32, i32]),
-        "erod_get_fused_config": (i32, [vp, P(i32), P(i32)]),
+        "erod_set_fused_config": (i32, [vp, i32, i32, i32]),
+        "erod_get_fused_config": (i32, [vp, P(i32), P(i32), P(i32)]),
+        "erod_autotune_fused": (i32, [vp, i32, i32, P(dbl)]),
         "erod_step_fused": (i32, [vp, i32, u64]),
         "erod_step_fused_graph": (i32, [vp, i32, u64, u64]),
         "erod_reorthonormalize": (i32, [vp]),
@@ -272,13 +273,21 @@ class DeviceBlock:
         t = np.ascontiguousarray(np.broadcast_to(np.asarray(t, dtype=np.float64), (self.n_rods,)))
         _check(_LIB.erod_set_rod_time(self._h, _dp(t)))
 
-    def set_fused_config(self, lanes_per_tile=0, steps_per_launch=0):
-        _check(_LIB.erod_set_fused_config(self._h, int(lanes_per_tile), int(steps_per_launch)))
+    def set_fused_config(self, lanes_per_tile=0, elements_per_thread=0, steps_per_launch=0):
+        _check(_LIB.erod_set_fused_config(self._h, int(lanes_per_tile), int(elements_per_thread),
+                                          int(steps_per_launch)))
 
     def fused_config(self):
-        a, b = ctypes.c_int(), ctypes.c_int()
-        _check(_LIB.erod_get_fused_config(self._h, ctypes.byref(a), ctypes.byref(b)))
-        return a.value, b.value
+        """(lanes_per_tile, elements_per_thread, steps_per_launch) of the current plan."""
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(_LIB.erod_get_fused_config(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def autotune(self, precision=FP32_BULK, trial_steps=20):
+        """Pick the fastest tile shape on this batch; returns its ms per step."""
+        ms = ctypes.c_double()
+        _check(_LIB.erod_autotune_fused(self._h, int(precision), int(trial_steps), ctypes.byref(ms)))
+        return ms.value
 
     def step_fused(self, n, precision=FP64):
         _check(_LIB.erod_step_fused(self._h, int(precision), int(n)))

@@ -24,7 +24,7 @@ COMMON = ["-O3", "-std=c++20", "-fPIC", "--offload-arch=" + ARCH,
 SOURCES = {
     "erod_ops.hip": ["-ffp-contract=off"],
     "erod_fused64.hip": ["-ffp-contract=off"],
-    "erod_fused32.hip": ["-ffp-contract=fast"],
+    "erod_fused32.hip": ["-ffp-contract=off"],
     "erod_fused.hip": [],
     "erod_capi.hip": [],
 }
@@ -54,6 +54,13 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    # the C++ host-layer driver (links liberod.so next to it)
+    drv = os.path.join(HERE, "timoshenko")
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++20", "-O2", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+           "-o", drv, os.path.join(CSRC, "host", "timoshenko.cpp"), "-L", HERE, "-lerod", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
     return LIB
 
 

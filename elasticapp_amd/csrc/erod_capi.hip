@@ -402,22 +402,6 @@ int erod_set_rod_time(erod_batch* b, const double* t) {
   return EROD_OK;
 }
 
-int erod_set_fused_config(erod_batch* b, int lanes_per_tile, int steps_per_launch) {
-  if (check_batch(b) || steps_per_launch < 0) return EROD_ERR_INVALID;
-  b->plan.lanes_per_tile = lanes_per_tile;
-  b->plan.steps_per_launch = steps_per_launch;
-  b->plan.valid = false;
-  b->graph_precision = -1;
-  return EROD_OK;
-}
-
-int erod_get_fused_config(const erod_batch* b, int* lanes, int* steps) {
-  if (check_batch(b)) return EROD_ERR_INVALID;
-  if (lanes) *lanes = b->plan.chosen_lanes;
-  if (steps) *steps = b->plan.steps_per_launch > 0 ? b->plan.steps_per_launch : 100;
-  return EROD_OK;
-}
-
 static int fused_common(erod_batch* b, int precision) {
   if (precision != EROD_FP64 && precision != EROD_FP32_BULK) {
     set_error("unknown precision");
@@ -431,6 +415,35 @@ static int fused_common(erod_batch* b, int precision) {
   int rc = ensure_damping(b);
   if (rc) return rc;
   return fused_prepare(b, precision);
+}
+
+int erod_set_fused_config(erod_batch* b, int lanes_per_tile, int elements_per_thread, int steps_per_launch) {
+  if (check_batch(b) || steps_per_launch < 0 || lanes_per_tile < 0 || elements_per_thread < 0) {
+    set_error("erod_set_fused_config: negative argument");
+    return EROD_ERR_INVALID;
+  }
+  b->plan.lanes_per_tile = lanes_per_tile;
+  b->plan.elements_per_thread = elements_per_thread;
+  b->plan.steps_per_launch = steps_per_launch;
+  b->plan.valid = false;
+  b->graph_precision = -1;
+  return EROD_OK;
+}
+
+int erod_get_fused_config(const erod_batch* b, int* lanes, int* elements_per_thread, int* steps) {
+  if (check_batch(b)) return EROD_ERR_INVALID;
+  const int si = b->plan.valid ? b->plan.shape : -1;
+  if (lanes) *lanes = si >= 0 ? fused_shape_lanes(si) : 0;
+  if (elements_per_thread) *elements_per_thread = si >= 0 ? fused_shape_epl(si) : 0;
+  if (steps) *steps = b->plan.steps_per_launch > 0 ? b->plan.steps_per_launch : 100;
+  return EROD_OK;
+}
+
+int erod_autotune_fused(erod_batch* b, int precision, int trial_steps, double* best_ms_per_step) {
+  if (check_batch(b) || trial_steps <= 0) return EROD_ERR_INVALID;
+  int rc = fused_common(b, precision);
+  if (rc) return rc;
+  return fused_autotune(b, precision, trial_steps, best_ms_per_step);
 }
 
 int erod_step_fused(erod_batch* b, int precision, uint64_t n) {

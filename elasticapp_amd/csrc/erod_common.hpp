@@ -73,11 +73,11 @@ namespace erod {
   };
 
   struct FusedPlan {
-    int lanes_per_tile = 0;       // requested (0 = autotune)
-    int steps_per_launch = 0;     // requested (0 = autotune)
+    int lanes_per_tile = 0;       // requested elements per tile (0 = any)
+    int elements_per_thread = 0;  // requested elements per thread (0 = any)
+    int steps_per_launch = 0;     // requested (0 = 100)
+    int shape = -1;               // chosen index into fused::kShapes
     int chosen_lanes = 0;
-    int chosen_steps = 0;
-    int epl = 1;                  // elements per lane
     uint64_t n_tiles = 0;
     uint32_t* d_tile_rods = nullptr;  // rod ids in tile order
     uint32_t* d_tile_lane0 = nullptr; // first lane of each rod within its tile
@@ -164,6 +164,9 @@ namespace erod {
   int fused_launch(erod_batch* b, int precision, int steps, bool timed);
   int fused_reorthonormalize(erod_batch* b);
   int fused_check_finite(erod_batch* b, uint8_t* flags);
+  int fused_autotune(erod_batch* b, int precision, int trial_steps, double* best_ms);
+  int fused_shape_lanes(int shape);
+  int fused_shape_epl(int shape);
   int sync_rods(erod_batch* b);
 
 }  // namespace erod

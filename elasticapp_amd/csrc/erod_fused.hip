@@ -72,68 +72,103 @@ namespace erod {
 
   // ---- planning: bin-pack rods into tiles --------------------------------------
 
+  int fused_shape_lanes(int si) { return fused::kShapes[si].P * fused::kShapes[si].NT; }
+  int fused_shape_epl(int si) { return fused::kShapes[si].P; }
+
+  // Feasible shapes: capacity P * NT holds the longest rod, and every rod has
+  // at least P elements (a thread then holds at most one rod end).
+  static bool shape_feasible(const erod_batch* b, int si) {
+    uint64_t maxn = 0, minn = ~uint64_t(0);
+    for (auto n : b->n_elems) {
+      maxn = n > maxn ? n : maxn;
+      minn = n < minn ? n : minn;
+    }
+    const fused::Shape sh = fused::kShapes[si];
+    const int lanes = b->plan.lanes_per_tile, epl = b->plan.elements_per_thread;
+    if (lanes && sh.P * sh.NT != lanes) return false;
+    if (epl && sh.P != epl) return false;
+    return uint64_t(sh.P) * sh.NT >= maxn && minn >= uint64_t(sh.P);
+  }
+
+  // Default shape without measurement: the smallest capacity that holds the
+  // longest rod; one element per thread up to 512 lanes, four elements per
+  // thread (ILP, one wave per SIMD, no register spills) for 1024.
+  static int default_shape(const erod_batch* b) {
+    static const int pref[] = {0, 1, 2, 3, 7, 5, 4, 6};
+    for (int si : pref)
+      if (shape_feasible(b, si)) return si;
+    return -1;
+  }
+
+  // First-fit-decreasing bin packing of the rods into tiles of P * NT elements.
+  static int plan_tiles(erod_batch* b, int si) {
+    const uint64_t cap = uint64_t(fused::kShapes[si].P) * fused::kShapes[si].NT;
+    std::vector<uint32_t> order(b->n_rods);
+    for (uint32_t i = 0; i < b->n_rods; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return b->n_elems[a] > b->n_elems[c]; });
+    std::vector<std::vector<uint32_t>> bins;
+    std::vector<uint64_t> fill;
+    for (uint32_t r : order) {
+      const uint64_t n = b->n_elems[r];
+      size_t bi = bins.size();
+      // the open bins are the last few; older ones are (nearly) full
+      for (size_t i = (bins.size() > 64 ? bins.size() - 64 : 0); i < bins.size(); ++i)
+        if (fill[i] + n <= cap) {
+          bi = i;
+          break;
+        }
+      if (bi == bins.size()) {
+        bins.emplace_back();
+        fill.push_back(0);
+      }
+      bins[bi].push_back(r);
+      fill[bi] += n;
+    }
+    std::vector<Tile> tiles(bins.size());
+    std::vector<uint32_t> trods, tlane0;
+    for (size_t i = 0; i < bins.size(); ++i) {
+      tiles[i].first = static_cast<uint32_t>(trods.size());
+      tiles[i].count = static_cast<uint32_t>(bins[i].size());
+      uint32_t lane = 0;
+      for (uint32_t r : bins[i]) {
+        trods.push_back(r);
+        tlane0.push_back(lane);
+        lane += static_cast<uint32_t>(b->n_elems[r]);
+      }
+      tiles[i].lanes = lane;
+      tiles[i].pad = 0;
+    }
+    if (b->plan.d_tiles) EROD_HIP(hipFree(b->plan.d_tiles));
+    if (b->plan.d_tile_rods) EROD_HIP(hipFree(b->plan.d_tile_rods));
+    if (b->plan.d_tile_lane0) EROD_HIP(hipFree(b->plan.d_tile_lane0));
+    b->plan.d_tiles = nullptr;
+    b->plan.d_tile_rods = b->plan.d_tile_lane0 = nullptr;
+    EROD_HIP(hipMalloc(&b->plan.d_tiles, tiles.size() * sizeof(Tile)));
+    EROD_HIP(hipMalloc(&b->plan.d_tile_rods, trods.size() * sizeof(uint32_t)));
+    EROD_HIP(hipMalloc(&b->plan.d_tile_lane0, tlane0.size() * sizeof(uint32_t)));
+    EROD_HIP(hipMemcpyAsync(b->plan.d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, b->stream));
+    EROD_HIP(hipMemcpyAsync(b->plan.d_tile_rods, trods.data(), trods.size() * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
+    EROD_HIP(hipMemcpyAsync(b->plan.d_tile_lane0, tlane0.data(), tlane0.size() * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
+    EROD_HIP(hipStreamSynchronize(b->stream));
+    b->plan.n_tiles = tiles.size();
+    b->plan.shape = si;
+    b->plan.chosen_lanes = static_cast<int>(cap);
+    b->plan.valid = true;
+    b->graph_precision = -1;
+    return EROD_OK;
+  }
+
   int fused_prepare(erod_batch* b, int precision) {
     int rc = sync_rods(b);
     if (rc) return rc;
     if (!b->plan.valid) {
-      // tile capacity (elements) and elements per thread
-      int cap = b->plan.lanes_per_tile;
-      uint64_t maxn = 0;
-      for (auto n : b->n_elems) maxn = n > maxn ? n : maxn;
-      if (cap == 0) cap = maxn > 512 ? 1024 : (maxn > 256 ? 512 : 256);
-      if (cap != 64 && cap != 128 && cap != 256 && cap != 512 && cap != 1024) {
-        set_error("lanes_per_tile must be one of 64, 128, 256, 512, 1024");
-        return EROD_ERR_INVALID;
-      }
-      if (maxn > uint64_t(cap)) {
-        set_error("a rod is longer than the tile capacity of the fused path");
+      const int si = default_shape(b);
+      if (si < 0) {
+        set_error("no fused tile shape holds this batch (rods longer than 1024 elements, or a requested shape "
+                  "that does not fit)");
         return EROD_ERR_UNSUPPORTED;
       }
-      b->plan.epl = cap == 1024 ? 2 : 1;
-      b->plan.chosen_lanes = cap;
-      // first-fit decreasing by length
-      std::vector<uint32_t> order(b->n_rods);
-      for (uint32_t i = 0; i < b->n_rods; ++i) order[i] = i;
-      std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return b->n_elems[a] > b->n_elems[c]; });
-      std::vector<std::vector<uint32_t>> bins;
-      std::vector<uint64_t> fill;
-      // bins sorted by remaining space would be faster; rods per bin are few
-      for (uint32_t r : order) {
-        const uint64_t n = b->n_elems[r];
-        size_t bi = bins.size();
-        for (size_t i = (bins.size() > 64 ? bins.size() - 64 : 0); i < bins.size(); ++i)
-          if (fill[i] + n <= uint64_t(cap)) { bi = i; break; }
-        if (bi == bins.size()) { bins.emplace_back(); fill.push_back(0); }
-        bins[bi].push_back(r);
-        fill[bi] += n;
-      }
-      std::vector<Tile> tiles(bins.size());
-      std::vector<uint32_t> trods, tlane0;
-      for (size_t i = 0; i < bins.size(); ++i) {
-        tiles[i].first = static_cast<uint32_t>(trods.size());
-        tiles[i].count = static_cast<uint32_t>(bins[i].size());
-        uint32_t lane = 0;
-        for (uint32_t r : bins[i]) {
-          trods.push_back(r);
-          tlane0.push_back(lane);
-          lane += static_cast<uint32_t>(b->n_elems[r]);
-        }
-        tiles[i].lanes = lane;
-        tiles[i].pad = 0;
-      }
-      hipFree(b->plan.d_tiles);
-      hipFree(b->plan.d_tile_rods);
-      hipFree(b->plan.d_tile_lane0);
-      EROD_HIP(hipMalloc(&b->plan.d_tiles, tiles.size() * sizeof(Tile)));
-      EROD_HIP(hipMalloc(&b->plan.d_tile_rods, trods.size() * sizeof(uint32_t)));
-      EROD_HIP(hipMalloc(&b->plan.d_tile_lane0, tlane0.size() * sizeof(uint32_t)));
-      EROD_HIP(hipMemcpyAsync(b->plan.d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, b->stream));
-      EROD_HIP(hipMemcpyAsync(b->plan.d_tile_rods, trods.data(), trods.size() * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
-      EROD_HIP(hipMemcpyAsync(b->plan.d_tile_lane0, tlane0.data(), tlane0.size() * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
-      EROD_HIP(hipStreamSynchronize(b->stream));
-      b->plan.n_tiles = tiles.size();
-      b->plan.chosen_steps = b->plan.steps_per_launch > 0 ? b->plan.steps_per_launch : 100;
-      b->plan.valid = true;
+      if ((rc = plan_tiles(b, si))) return rc;
     }
     if (precision == EROD_FP32_BULK && b->const32_dirty) {
       const uint64_t total = const32_offset(b, kNConst32);
@@ -189,6 +224,60 @@ namespace erod {
     C.kap0 = p + const32_offset(b, 8);
     C.B = p + const32_offset(b, 9);
     return C;
+  }
+
+  // Measured choice of the tile shape: every feasible shape advances the
+  // batch `trial_steps` steps from the same snapshot of x, v, Q, w and the rod
+  // clocks (restored afterwards), timed with HIP events; the fastest is kept.
+  int fused_autotune(erod_batch* b, int precision, int trial_steps, double* best_ms) {
+    int rc = fused_prepare(b, precision);
+    if (rc) return rc;
+    const int snap_fields[4] = {EROD_POSITION, EROD_VELOCITY, EROD_DIRECTOR, EROD_ANGULAR_VELOCITY};
+    std::vector<double*> snap(5, nullptr);
+    std::vector<size_t> bytes(5);
+    for (int i = 0; i < 4; ++i) bytes[i] = sizeof(double) * field_info(snap_fields[i]).ncomp * b->field_slots(snap_fields[i]);
+    bytes[4] = sizeof(double) * b->n_rods;
+    auto dev = [&](int i) { return i < 4 ? b->d_field[snap_fields[i]] : b->d_time; };
+    for (int i = 0; i < 5; ++i) {
+      EROD_HIP(hipMalloc(&snap[i], bytes[i]));
+      EROD_HIP(hipMemcpyAsync(snap[i], dev(i), bytes[i], hipMemcpyDeviceToDevice, b->stream));
+    }
+    auto restore = [&]() -> int {
+      for (int i = 0; i < 5; ++i) EROD_HIP(hipMemcpyAsync(dev(i), snap[i], bytes[i], hipMemcpyDeviceToDevice, b->stream));
+      return EROD_OK;
+    };
+    const int keep_lanes = b->plan.lanes_per_tile, keep_epl = b->plan.elements_per_thread;
+    b->plan.lanes_per_tile = 0;
+    b->plan.elements_per_thread = 0;
+    int best = -1;
+    double best_t = 0.0;
+    for (int si = 0; si < fused::kNShapes && rc == EROD_OK; ++si) {
+      if (!shape_feasible(b, si)) continue;
+      if ((rc = plan_tiles(b, si))) break;
+      if ((rc = fused_launch(b, precision, trial_steps, false))) break;  // warm
+      if ((rc = fused_launch(b, precision, trial_steps, true))) break;
+      float ms = 0.0f;
+      if ((rc = hip_check(hipEventSynchronize(b->ev1), "autotune sync"))) break;
+      if ((rc = hip_check(hipEventElapsedTime(&ms, b->ev0, b->ev1), "autotune time"))) break;
+      if (best < 0 || ms < best_t) {
+        best = si;
+        best_t = ms;
+      }
+      if ((rc = restore())) break;
+    }
+    b->plan.lanes_per_tile = keep_lanes;
+    b->plan.elements_per_thread = keep_epl;
+    if (rc == EROD_OK && best >= 0) rc = plan_tiles(b, best);
+    if (rc == EROD_OK) rc = restore();
+    if (rc == EROD_OK) rc = hip_check(hipStreamSynchronize(b->stream), "autotune");
+    for (double* p : snap)
+      if (p) (void)hipFree(p);
+    if (rc == EROD_OK && best < 0) {
+      set_error("no feasible fused tile shape");
+      rc = EROD_ERR_UNSUPPORTED;
+    }
+    if (best_ms) *best_ms = best_t / trial_steps;
+    return rc;
   }
 
   int fused_launch(erod_batch* b, int precision, int steps, bool timed) {

@@ -1,4 +1,6 @@
-// FP32-bulk instantiation of the fused step kernel (FMA contraction allowed).
+// FP32-bulk instantiation of the fused step kernel (built with
+// -ffp-contract=off: the fused multiply-adds are the explicit ones of the
+// kernel source, so results do not depend on the tile shape).
 #include "erod_fused_kernel.hpp"
 
 namespace erod {

@@ -1,15 +1,15 @@
-#pragma once
 // Fused, persistent position-Verlet step kernel (device code, instantiated
 // for FP64 in erod_fused64.hip and for FP32-bulk in erod_fused32.hip).
 //
 // One workgroup owns a tile of whole rods.  Each thread holds P contiguous
 // elements of the tile (element k, node k and, for the last element of a rod,
 // the rod's last node n) with the state x, v, Q, w in FP64 registers across
-// `nsteps` steps; only the constants are re-read each step (they stay in L2).
-// The only cross-thread data of a step are the +1/+2 neighbour node values,
-// the +1 neighbour director (first exchange) and the -1 neighbour element
-// force / couple terms (second exchange), passed through two LDS arrays, so a
-// step costs two workgroup barriers and no HBM traffic for the state.
+// `nsteps` steps.  The only cross-thread data of a step are the +1/+2
+// neighbour node values, the +1 neighbour director (first exchange) and the
+// -1 neighbour element force / couple terms (second exchange), passed through
+// two LDS arrays, so a step costs two workgroup barriers and no HBM traffic
+// for the state.  P > 1 gives each thread several independent elements (ILP)
+// and lets rods of up to P * NT elements share one workgroup.
 //
 // Arithmetic of one step follows the reference block timestep
 // (PositionVerlet.hpp:49-88, SymplecticSteppers.hpp:152-216) with explicit rod
@@ -22,17 +22,22 @@
 //   accelerations     WithRodKinematics.hpp:683-714
 //   director update   SO3PrimitiveAddAssign/Scalar.hpp:25-96
 //
-// Precision: R = double keeps the reference formulas (acos-based log map with
-// its 1e-10 / 1e-14 offsets), its divisions and its order of evaluation, and
-// is compiled without FMA contraction, so that it rounds like the reference.  R = float is the FP32-bulk path: FP64 state,
-// FP64 position/velocity/angular-velocity accumulation and FP64 exponential
-// map; strains, stresses, couples, loads and torques in FP32, with the
-// cancellation-prone parts (node differences, the relative rotation of
-// neighbouring directors) formed in FP64 before rounding, and the log map
-// rewritten as theta = 2 asin(sqrt(sin^2(theta/2) + 5e-11)), which is the
-// reference's acos(cos(theta) - 1e-10) without its cancellation.
-#include "erod_common.hpp"
+// Precision
+//   R = double: the reference formulas (acos-based log map with its 1e-10 /
+//   1e-14 offsets, library sin / cos in the director update), its divisions
+//   and its order of evaluation; built without FMA contraction, it rounds
+//   like the op-by-op port (and equals it bitwise on the test batches).
+//   R = float (FP32-bulk): FP64 state, FP64 accumulation of x, v, w and an
+//   FP64 exponential map (the series form below, which needs no sqrt or
+//   division); strains, stresses, couples, loads and torques in FP32 with the
+//   cancellation-prone parts (node differences and the relative rotation of
+//   neighbouring directors) formed in FP64 before rounding.  The log map is
+//   the reference's acos(cos(theta) - 1e-10) rewritten without cancellation
+//   as theta = 2 asin(sqrt(sin^2(theta/2) + 5e-11)).  The constants are
+//   held in FP32 registers for the whole launch.
+#pragma once
 
+#include "erod_common.hpp"
 
 namespace erod {
   namespace fused {
@@ -59,41 +64,56 @@ namespace erod {
     __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
     __device__ __forceinline__ float div_(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     __device__ __forceinline__ double div_(double a, double b) { return a / b; }
+
+    // a * b + c: one fused multiply-add in the FP32-bulk path, a separately
+    // rounded product and sum in the FP64 path (which keeps the reference's
+    // rounding).  The units are built without automatic contraction, so every
+    // FMA is one of these and the arithmetic of an element does not depend on
+    // the surrounding code (tile shape, position within a thread).
+    template <typename R>
+    __device__ __forceinline__ double maddd(double a, double b, double c) {
+      if constexpr (sizeof(R) == 8) return a * b + c;
+      else return __builtin_fma(a, b, c);
+    }
+    __device__ __forceinline__ float madd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+    __device__ __forceinline__ double madd(double a, double b, double c) { return a * b + c; }
+
     // element length from an FP64 node difference; both neighbours of a
     // Voronoi region evaluate it the same way
     template <typename R>
     __device__ __forceinline__ R len_(double d0, double d1, double d2) {
       if constexpr (sizeof(R) == 8) return sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-      else return sqrt_(static_cast<float>(d0 * d0 + d1 * d1 + d2 * d2));
+      else return sqrt_(static_cast<float>(__builtin_fma(d2, d2, __builtin_fma(d1, d1, d0 * d0))));
     }
 
-    // sin / cos of a rotation angle in FP64.  The director increments of one
-    // half step are tiny (|w| dt / 2), where the Taylor series to x^9 / x^8 is
-    // exact to rounding; larger angles use the library functions.
-    __device__ __forceinline__ void sincos_small(double th, double& s, double& c) {
-      if (th < 1.0e-2) {
-        const double t2 = th * th;
-        s = th * (1.0 - t2 * (1.0 / 6.0) * (1.0 - t2 * (1.0 / 20.0) * (1.0 - t2 * (1.0 / 42.0) * (1.0 - t2 * (1.0 / 72.0)))));
-        c = 1.0 - t2 * 0.5 * (1.0 - t2 * (1.0 / 12.0) * (1.0 - t2 * (1.0 / 30.0) * (1.0 - t2 * (1.0 / 56.0))));
-      } else {
-        sincos(th, &s, &c);
-      }
-    }
-
-    // Q <- exp(-[u]x) Q with u = c w (SO3AddAssign, Scalar.hpp:25-96), FP64.
-    // The FP64 path (R = double) calls the library sin / cos like the
-    // reference; the FP32-bulk path uses the small-angle series.
     template <typename R>
-    __device__ __forceinline__ void rodrigues(double q[9], const double w[3], double cdt) {
+    __device__ __forceinline__ void cross3(const R a[3], const R b[3], R o[3]) {
+      o[0] = madd(a[1], b[2], -(a[2] * b[1]));
+      o[1] = madd(a[2], b[0], -(a[0] * b[2]));
+      o[2] = madd(a[0], b[1], -(a[1] * b[0]));
+    }
+
+    __device__ __forceinline__ double ramp_value(int kind, double t, double start, double interval) {
+      if (kind == EROD_RAMP_NONE) return 1.0;
+      if (t < start) return 0.0;
+      if (t > start + interval) return 1.0;
+      if (kind == EROD_RAMP_LINEAR) return (t - start) / interval;
+      const double s = 2.0 * ((t - start) / interval) - 1.0;
+      return 0.5 * (s + (1.0 + (sin(M_PI * s) / M_PI)));
+    }
+
+    // ---- director update Q <- exp(-[u]x) Q, u = c w, in FP64 ----------------------
+
+    template <typename R>
+    __device__ __forceinline__ void rodrigues(double q[9], const double w[3], double cdt);
+
+    // the reference (SO3AddAssign<scalar>, Scalar.hpp:25-96)
+    template <>
+    __device__ __forceinline__ void rodrigues<double>(double q[9], const double w[3], double cdt) {
       double u0 = cdt * w[0], u1 = cdt * w[1], u2 = cdt * w[2];
       const double theta = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
-      double beta, c_alpha;
-      if constexpr (sizeof(R) == 8) {
-        beta = sin(theta);
-        c_alpha = cos(theta);
-      } else {
-        sincos_small(theta, beta, c_alpha);
-      }
+      const double beta = sin(theta);
+      const double c_alpha = cos(theta);
       const double alpha = 1.0 - c_alpha;
       const double den = theta + 1e-14;
       u0 /= den;
@@ -109,23 +129,40 @@ namespace erod {
       }
     }
 
-    template <typename R>
-    __device__ __forceinline__ void cross3(const R a[3], const R b[3], R o[3]) {
-      o[0] = a[1] * b[2] - a[2] * b[1];
-      o[1] = a[2] * b[0] - a[0] * b[2];
-      o[2] = a[0] * b[1] - a[1] * b[0];
+    // FP32-bulk: the same map written in u itself,
+    //   q' = q - th^2 B q + B (u . q) u + A (q x u),
+    // A = sin(th)/th, B = (1 - cos(th))/th^2, both even series in th^2 = u . u
+    // (exact to rounding for th < 1e-2), so no sqrt or division per step.
+    template <>
+    __device__ __forceinline__ void rodrigues<float>(double q[9], const double w[3], double cdt) {
+      const double u0 = cdt * w[0], u1 = cdt * w[1], u2 = cdt * w[2];
+      const double t2 = __builtin_fma(u2, u2, __builtin_fma(u1, u1, u0 * u0));
+      double A, B;
+      if (t2 < 1.0e-4) {
+        // A = 1 - t2/6 + t2^2/120 - t2^3/5040, B = 1/2 - t2/24 + t2^2/720 - t2^3/40320
+        A = __builtin_fma(t2, __builtin_fma(t2, __builtin_fma(t2, -1.0 / 5040.0, 1.0 / 120.0), -1.0 / 6.0), 1.0);
+        B = __builtin_fma(t2, __builtin_fma(t2, __builtin_fma(t2, -1.0 / 40320.0, 1.0 / 720.0), -1.0 / 24.0), 0.5);
+      } else {
+        const double th = sqrt(t2);
+        double s, c, sh, ch;
+        sincos(th, &s, &c);
+        sincos(0.5 * th, &sh, &ch);
+        A = s / th;
+        B = 2.0 * sh * sh / t2;
+      }
+      const double cq = __builtin_fma(-t2, B, 1.0);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double a = q[j], b = q[3 + j], cc = q[6 + j];
+        const double bu = B * __builtin_fma(cc, u2, __builtin_fma(b, u1, a * u0));
+        q[j] = __builtin_fma(cq, a, __builtin_fma(bu, u0, A * __builtin_fma(b, u2, -(cc * u1))));
+        q[3 + j] = __builtin_fma(cq, b, __builtin_fma(bu, u1, A * __builtin_fma(cc, u0, -(a * u2))));
+        q[6 + j] = __builtin_fma(cq, cc, __builtin_fma(bu, u2, A * __builtin_fma(a, u1, -(b * u0))));
+      }
     }
 
-    __device__ __forceinline__ double ramp_value(int kind, double t, double start, double interval) {
-      if (kind == EROD_RAMP_NONE) return 1.0;
-      if (t < start) return 0.0;
-      if (t > start + interval) return 1.0;
-      if (kind == EROD_RAMP_LINEAR) return (t - start) / interval;
-      const double s = 2.0 * ((t - start) / interval) - 1.0;
-      return 0.5 * (s + (1.0 + (sin(M_PI * s) / M_PI)));
-    }
+    // ---- curvature of the Voronoi region between directors qc and qn / D0 --------
 
-    // curvature of the Voronoi region between directors qc and qn, divided by D0
     template <typename R>
     __device__ __forceinline__ void curvature(const double qc[9], const double qn[9], R D0, R kap[3]);
 
@@ -141,23 +178,81 @@ namespace erod {
       kap[2] = st * (-qc[0] * qn[3] - qc[1] * qn[4] - qc[2] * qn[5] + qc[3] * qn[0] + qc[4] * qn[1] + qc[5] * qn[2]);
     }
 
-    // FP32-bulk: the same quantity without the cancellation of acos near 1
+    // FP32-bulk: skew part and trace in FP64, prefactor in FP32.  With
+    // a = sin^2(theta/2) (3 - tr)/4 + 5e-11, the reference prefactor
+    // theta / (2 sin theta) equals asin(sqrt a) / (2 sqrt(a) sqrt(1 - a)); the
+    // first factor is the series 1 + a/6 + 3a^2/40 + 5a^3/112 + 35a^4/1152
+    // for a < 0.02 (theta < 16 degrees) and the library asin above.
+    __device__ __forceinline__ double dot3_fma(const double* a, const double* b) {
+      return __builtin_fma(a[2], b[2], __builtin_fma(a[1], b[1], a[0] * b[0]));
+    }
+    // sum_j a_j b_j - c_j d_j over three j, with FMAs
+    __device__ __forceinline__ double skew3_fma(const double* a, const double* b, const double* c, const double* d) {
+      double s = a[0] * b[0];
+      s = __builtin_fma(a[1], b[1], s);
+      s = __builtin_fma(a[2], b[2], s);
+      s = __builtin_fma(-c[0], d[0], s);
+      s = __builtin_fma(-c[1], d[1], s);
+      return __builtin_fma(-c[2], d[2], s);
+    }
+
     template <>
     __device__ __forceinline__ void curvature<float>(const double qc[9], const double qn[9], float D0, float kap[3]) {
-      const double tr = qc[0] * qn[0] + qc[1] * qn[1] + qc[2] * qn[2] + qc[3] * qn[3] + qc[4] * qn[4] +
-                        qc[5] * qn[5] + qc[6] * qn[6] + qc[7] * qn[7] + qc[8] * qn[8];
-      // 2 sin(theta) * axis
-      const double s0 = -qc[3] * qn[6] - qc[4] * qn[7] - qc[5] * qn[8] + qc[6] * qn[3] + qc[7] * qn[4] + qc[8] * qn[5];
-      const double s1 = qc[0] * qn[6] + qc[1] * qn[7] + qc[2] * qn[8] - qc[6] * qn[0] - qc[7] * qn[1] - qc[8] * qn[2];
-      const double s2 = -qc[0] * qn[3] - qc[1] * qn[4] - qc[2] * qn[5] + qc[3] * qn[0] + qc[4] * qn[1] + qc[5] * qn[2];
-      // sin^2(theta/2) = (3 - tr) / 4; the reference's -1e-10 inside acos adds 5e-11
+      const double tr = dot3_fma(qc, qn) + dot3_fma(qc + 3, qn + 3) + dot3_fma(qc + 6, qn + 6);
+      // rows: d1 = q[0..2], d2 = q[3..5], d3 = q[6..8]
+      const double s0 = skew3_fma(qc + 6, qn + 3, qc + 3, qn + 6);
+      const double s1 = skew3_fma(qc + 0, qn + 6, qc + 6, qn + 0);
+      const double s2 = skew3_fma(qc + 3, qn + 0, qc + 0, qn + 3);
       float a = static_cast<float>((3.0 - tr) * 0.25) + 5e-11f;
-      a = fminf(fmaxf(a, 0.0f), 1.0f);
-      const float theta = 2.0f * asinf(sqrt_(a));
-      const float st = div_(0.5f * theta, sinf(theta + 1e-14f) * D0);
+      a = fminf(fmaxf(a, 0.0f), 0.999999f);
+      float f;
+      if (a < 0.02f) {
+        f = madd(a, madd(a, madd(a, madd(a, 35.0f / 1152.0f, 5.0f / 112.0f), 3.0f / 40.0f), 1.0f / 6.0f), 1.0f);
+      } else {
+        const float sh = sqrt_(a);
+        f = asinf(sh) / sh;
+      }
+      const float st = f * rcp_(2.0f * sqrt_(1.0f - a) * D0);
       kap[0] = st * static_cast<float>(s0);
       kap[1] = st * static_cast<float>(s1);
       kap[2] = st * static_cast<float>(s2);
+    }
+
+    // ---- per element constants ------------------------------------------------
+
+    template <typename R>
+    struct ElemConst {
+      R m, invm, mn;  // node k mass and inverse, node k+1 mass
+      R l0, sig0[3], S[3], J[3], iJ[3];
+      R D0, kap0[3], B[3];  // Voronoi k (unused at the rod end)
+    };
+
+    template <typename R>
+    __device__ __forceinline__ void load_const(const Consts<R>& C, uint64_t s, uint64_t E, bool has_vor, ElemConst<R>& K) {
+      const uint64_t V = E - 1;
+      K.m = C.m[s];
+      K.invm = C.invm[s];
+      K.mn = C.m[s + 1];
+      K.l0 = C.l0[s];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        K.sig0[c] = C.sig0[c * E + s];
+        K.S[c] = C.S[c * E + s];
+        K.J[c] = C.J[c * E + s];
+        K.iJ[c] = C.iJ[c * E + s];
+      }
+      if (has_vor) {
+        K.D0 = C.D0[s];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          K.kap0[c] = C.kap0[c * V + s];
+          K.B[c] = C.B[c * V + s];
+        }
+      } else {
+        K.D0 = R(1);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) K.kap0[c] = K.B[c] = R(0);
+      }
     }
 
     // per-element values carried from the first to the second half of a step
@@ -173,13 +268,13 @@ namespace erod {
     };
 
     // Element k (and Voronoi k when it exists) of a rod: everything that needs
-    // only the +1 / +2 neighbours.
+    // only the +1 / +2 neighbours.  qr is the director used for Q t and Q^T n
+    // (the FP64 state for R = double, its FP32 copy for R = float).
     template <typename R>
     __device__ __forceinline__ void element_forces(const double xk[3], const double xk1[3], const double xk2[3],
                                                    const double vk[3], const double vk1[3], const double qk[9],
-                                                   const double qk1[9], const double wk[3], bool has_vor,
-                                                   const Consts<R>& C, uint64_t es, uint64_t E, Mid<R>& out) {
-      const uint64_t V = E - 1;
+                                                   const R qr[9], const double qk1[9], const double wk[3], bool has_vor,
+                                                   const ElemConst<R>& K, Mid<R>& out) {
       // spanwise geometry: node difference in FP64, then R
       const double d0 = xk1[0] - xk[0], d1 = xk1[1] - xk[1], d2 = xk1[2] - xk[2];
       const R l = len_<R>(d0, d1, d2);
@@ -194,29 +289,29 @@ namespace erod {
         t[1] = static_cast<float>(d1) * il;
         t[2] = static_cast<float>(d2) * il;
       }
-      const R l0 = C.l0[es];
-      const R e = div_(l, l0);
-      // shear / stretch strain sigma = e (Q t) - z; Q t formed in FP64
+      const R e = div_(l, K.l0);
+      // shear / stretch strain sigma = e (Q t) - z
       R qt[3];
 #pragma unroll
-      for (int p = 0; p < 3; ++p)
-        qt[p] = static_cast<R>(qk[3 * p] * static_cast<double>(t[0]) + qk[3 * p + 1] * static_cast<double>(t[1]) +
-                               qk[3 * p + 2] * static_cast<double>(t[2]));
+      for (int p = 0; p < 3; ++p) qt[p] = madd(qr[3 * p + 2], t[2], madd(qr[3 * p + 1], t[1], qr[3 * p] * t[0]));
       R n[3];
 #pragma unroll
       for (int p = 0; p < 3; ++p) {
-        R sig = qt[p] * e;
-        if (p == 2) sig -= R(1);
-        n[p] = C.S[p * E + es] * (sig - C.sig0[p * E + es]);
+        R sig = p == 2 ? madd(qt[p], e, R(-1)) : qt[p] * e;
+        n[p] = K.S[p] * (sig - K.sig0[p]);
       }
       // Q^T n / e
       R qn[3];
 #pragma unroll
-      for (int j = 0; j < 3; ++j)
-        qn[j] = static_cast<R>(qk[j] * static_cast<double>(n[0]) + qk[3 + j] * static_cast<double>(n[1]) +
-                               qk[6 + j] * static_cast<double>(n[2]));
+      for (int j = 0; j < 3; ++j) qn[j] = madd(qr[6 + j], n[2], madd(qr[3 + j], n[1], qr[j] * n[0]));
+      if constexpr (sizeof(R) == 8) {
 #pragma unroll
-      for (int j = 0; j < 3; ++j) out.c[j] = div_(qn[j], e);
+        for (int j = 0; j < 3; ++j) out.c[j] = qn[j] / e;
+      } else {
+        const float ie = rcp_(e);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) out.c[j] = qn[j] * ie;
+      }
       out.e = e;
       // local torque terms: (Q t x n) l0 + (J w / e) x w + (J w / e) (t . dv) / (e l0)
       R sh[3];
@@ -224,14 +319,14 @@ namespace erod {
       R w[3] = {static_cast<R>(wk[0]), static_cast<R>(wk[1]), static_cast<R>(wk[2])};
       R jw[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) jw[c] = div_(C.J[c * E + es] * w[c], e);
+      for (int c = 0; c < 3; ++c) jw[c] = div_(K.J[c] * w[c], e);
       R lag[3];
       cross3(jw, w, lag);
       const R dv0 = static_cast<R>(vk1[0] - vk[0]), dv1 = static_cast<R>(vk1[1] - vk[1]), dv2 = static_cast<R>(vk1[2] - vk[2]);
-      const R rate = div_(t[0] * dv0 + t[1] * dv1 + t[2] * dv2, e * l0);
+      const R rate = div_(madd(t[2], dv2, madd(t[1], dv1, t[0] * dv0)), e * K.l0);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        out.shear[c] = sh[c] * l0;
+        out.shear[c] = sh[c] * K.l0;
         out.lag[c] = lag[c];
         out.unst[c] = jw[c] * rate;
       }
@@ -239,18 +334,17 @@ namespace erod {
       if (has_vor) {
         const double f0 = xk2[0] - xk1[0], f1 = xk2[1] - xk1[1], f2 = xk2[2] - xk1[2];
         const R l1 = len_<R>(f0, f1, f2);
-        const R D0 = C.D0[es];
         const R D = R(0.5) * (l + l1);
-        const R eps = div_(D, D0);
+        const R eps = div_(D, K.D0);
         R kap[3];
-        curvature<R>(qk, qk1, D0, kap);
+        curvature<R>(qk, qk1, K.D0, kap);
         R M[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) M[c] = C.B[c * V + es] * (kap[c] - C.kap0[c * V + es]);
+        for (int c = 0; c < 3; ++c) M[c] = K.B[c] * (kap[c] - K.kap0[c]);
         const R ic = rcp_(eps * eps * eps);
         R km[3];
         cross3(kap, M, km);
-        const R icd = ic * D0;
+        const R icd = ic * K.D0;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           out.P[c] = M[c] * ic;
@@ -286,12 +380,15 @@ namespace erod {
       }
     }
 
+    // ---- the kernel ------------------------------------------------------------------
+
     template <typename R, int P, int NT>
     __global__ void __launch_bounds__(NT) k_fused(double* __restrict__ X, double* __restrict__ Vel, double* __restrict__ Qd,
                                                   double* __restrict__ W, Consts<R> C, const RodInfo* __restrict__ rods,
                                                   const Tile* __restrict__ tiles, const uint32_t* __restrict__ tile_rods,
                                                   const uint32_t* __restrict__ tile_lane0, double* __restrict__ rod_time,
                                                   Env env, int nsteps, uint64_t E) {
+      constexpr bool kCache = sizeof(R) == 4;  // FP32 constants stay in registers for the launch
       const uint64_t N = E + 1;
       const Tile tile = tiles[blockIdx.x];
       const int tid = threadIdx.x;
@@ -310,6 +407,8 @@ namespace erod {
       double tm[P], dt[P];
       uint32_t rflags[P];
       R gfj[P], gtj[P];
+      ElemConst<R> kc[kCache ? P : 1];
+      R tail_m = R(0), tail_im = R(0);
       double xt[3] = {0, 0, 0}, vt[3] = {0, 0, 0};
       int tailj = -1;
 #pragma unroll
@@ -350,7 +449,10 @@ namespace erod {
               xt[c] = X[c * N + slot[j] + 1];
               vt[c] = Vel[c * N + slot[j] + 1];
             }
+            tail_m = C.m[slot[j] + 1];
+            tail_im = C.invm[slot[j] + 1];
           }
+          if constexpr (kCache) load_const<R>(C, slot[j], E, kk[j] < nn[j] - 1, kc[kCache ? j : 0]);
         } else {
 #pragma unroll
           for (int c = 0; c < 3; ++c) x[j][c] = v[j][c] = w[j][c] = 0.0;
@@ -366,11 +468,11 @@ namespace erod {
           if (!valid[j]) continue;
           const double ch = 0.5 * dt[j];
 #pragma unroll
-          for (int c = 0; c < 3; ++c) x[j][c] += ch * v[j][c];
+          for (int c = 0; c < 3; ++c) x[j][c] = maddd<R>(ch, v[j][c], x[j][c]);
           rodrigues<R>(q[j], w[j], ch);
           if (j == tailj)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) xt[c] += ch * vt[c];
+            for (int c = 0; c < 3; ++c) xt[c] = maddd<R>(ch, vt[c], xt[c]);
           tm[j] += 0.5 * dt[j];
           if (kk[j] == 0 && (rflags[j] & 1u)) {
             const RodInfo& ri = rods[rodi[j]];
@@ -461,7 +563,13 @@ namespace erod {
 #pragma unroll
             for (int m = 0; m < 9; ++m) qk1[m] = 0.0;
           }
-          element_forces<R>(x[j], xk1, xk2, v[j], vk1, q[j], qk1, w[j], has_vor, C, slot[j], E, mid[j]);
+          ElemConst<R> K;
+          if constexpr (kCache) K = kc[kCache ? j : 0];
+          else load_const<R>(C, slot[j], E, has_vor, K);
+          R qr[9];
+#pragma unroll
+          for (int m = 0; m < 9; ++m) qr[m] = static_cast<R>(q[j][m]);
+          element_forces<R>(x[j], xk1, xk2, v[j], vk1, q[j], qr, qk1, w[j], has_vor, K, mid[j]);
         }
         // ---- exchange 2 -----------------------------------------------------------
 #pragma unroll
@@ -497,7 +605,21 @@ namespace erod {
           }
           const uint64_t s = slot[j];
           const Mid<R>& md = mid[j];
-          const R mk = C.m[s];
+          R mk, mkn, im, iJ[3];
+          if constexpr (kCache) {
+            const ElemConst<R>& K = kc[kCache ? j : 0];
+            mk = K.m;
+            mkn = K.mn;
+            im = K.invm;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) iJ[c] = K.iJ[c];
+          } else {
+            mk = C.m[s];
+            mkn = C.m[s + 1];
+            im = C.invm[s];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) iJ[c] = C.iJ[c * E + s];
+          }
           const R gf = gfj[j], gt = gtj[j];
           // internal force on node k: difference of c, then damping
           R f[3];
@@ -506,7 +628,7 @@ namespace erod {
 #pragma unroll
           for (int c = 0; c < 3; ++c) f[c] = (first ? md.c[c] : md.c[c] - cm[c]) - mg * vk[c];
           // internal torque on element k
-          const R em = R(0.5) * (mk + C.m[s + 1]);
+          const R em = R(0.5) * (mk + mkn);
           const R eg = em * gt;
           R tau[3];
 #pragma unroll
@@ -518,22 +640,20 @@ namespace erod {
           // external loads of node k (forcing, then environment)
           double fe[3] = {0.0, 0.0, 0.0};
           node_environment(env, static_cast<double>(mk), x[j], v[j], fe);
-          const R im = C.invm[s];
           const double dtt = dt[j];
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
             const R a = (f[c] + static_cast<R>(fe[c])) * im;
-            v[j][c] += dtt * static_cast<double>(a);
+            v[j][c] = maddd<R>(dtt, static_cast<double>(a), v[j][c]);
           }
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            const R al = (C.iJ[c * E + s] * tau[c]) * md.e;
-            w[j][c] += dtt * static_cast<double>(al);
+            const R al = (iJ[c] * tau[c]) * md.e;
+            w[j][c] = maddd<R>(dtt, static_cast<double>(al), w[j][c]);
           }
           // last node of the rod
           if (last) {
-            const R mn = C.m[s + 1];
-            const R mgn = mn * gf;
+            const R mgn = tail_m * gf;
             double fen[3] = {0.0, 0.0, 0.0};
             if (rflags[j] & 2u) {
               const RodInfo& ri = rods[rodi[j]];
@@ -541,13 +661,12 @@ namespace erod {
 #pragma unroll
               for (int c = 0; c < 3; ++c) fen[c] = ri.force[c] * rv;
             }
-            node_environment(env, static_cast<double>(mn), xt, vt, fen);
-            const R imn = C.invm[s + 1];
+            node_environment(env, static_cast<double>(tail_m), xt, vt, fen);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
               const R fn = -md.c[c] - mgn * static_cast<R>(vt[c]);
-              const R a = (fn + static_cast<R>(fen[c])) * imn;
-              vt[c] += dtt * static_cast<double>(a);
+              const R a = (fn + static_cast<R>(fen[c])) * tail_im;
+              vt[c] = maddd<R>(dtt, static_cast<double>(a), vt[c]);
             }
           }
           // post-dynamic clamp
@@ -566,11 +685,11 @@ namespace erod {
           tm[j] += 0.5 * dt[j];
           const double ch = 0.5 * dt[j];
 #pragma unroll
-          for (int c = 0; c < 3; ++c) x[j][c] += ch * v[j][c];
+          for (int c = 0; c < 3; ++c) x[j][c] = maddd<R>(ch, v[j][c], x[j][c]);
           rodrigues<R>(q[j], w[j], ch);
           if (j == tailj)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) xt[c] += ch * vt[c];
+            for (int c = 0; c < 3; ++c) xt[c] = maddd<R>(ch, vt[c], xt[c]);
         }
       }
 
@@ -598,7 +717,6 @@ namespace erod {
       }
     }
 
-
     template <typename R, int P, int NT>
     inline int launch_tiles(erod_batch* b, const Consts<R>& C, int steps) {
       if (b->plan.n_tiles == 0) return EROD_OK;
@@ -609,16 +727,26 @@ namespace erod {
       return hip_check(hipGetLastError(), "k_fused");
     }
 
-    // one launch of `steps` steps over every tile, by tile capacity
+    // the tile shapes the planner can choose: elements per thread x threads
+    struct Shape {
+      int P, NT;
+    };
+    constexpr Shape kShapes[] = {{1, 64}, {1, 128}, {1, 256}, {1, 512}, {2, 256}, {2, 512}, {4, 128}, {4, 256}};
+    constexpr int kNShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+
+    // one launch of `steps` steps over every tile with the planned shape
     template <typename R>
     inline int launch_all(erod_batch* b, const Consts<R>& C, int steps) {
-      switch (b->plan.chosen_lanes) {
-        case 64: return launch_tiles<R, 1, 64>(b, C, steps);
-        case 128: return launch_tiles<R, 1, 128>(b, C, steps);
-        case 256: return launch_tiles<R, 1, 256>(b, C, steps);
-        case 512: return launch_tiles<R, 1, 512>(b, C, steps);
-        case 1024: return launch_tiles<R, 2, 512>(b, C, steps);
-        default: set_error("bad tile size"); return EROD_ERR_INVALID;
+      switch (b->plan.shape) {
+        case 0: return launch_tiles<R, 1, 64>(b, C, steps);
+        case 1: return launch_tiles<R, 1, 128>(b, C, steps);
+        case 2: return launch_tiles<R, 1, 256>(b, C, steps);
+        case 3: return launch_tiles<R, 1, 512>(b, C, steps);
+        case 4: return launch_tiles<R, 2, 256>(b, C, steps);
+        case 5: return launch_tiles<R, 2, 512>(b, C, steps);
+        case 6: return launch_tiles<R, 4, 128>(b, C, steps);
+        case 7: return launch_tiles<R, 4, 256>(b, C, steps);
+        default: set_error("bad tile shape"); return EROD_ERR_INVALID;
       }
     }
 

@@ -140,11 +140,18 @@ int erod_set_rod_dt(erod_batch* b, const double* dt_per_rod);
 /* Per-rod clock (seconds), read back after fused steps. */
 int erod_get_rod_time(erod_batch* b, double* time_per_rod);
 int erod_set_rod_time(erod_batch* b, const double* time_per_rod);
-/* Tile / launch shape of the persistent kernels.  lanes_per_tile in
- * {64,128,256,512,1024}; 0 lets the autotuner decide.  steps_per_launch: steps a
- * workgroup advances with its rods' state held on chip. */
-int erod_set_fused_config(erod_batch* b, int lanes_per_tile, int steps_per_launch);
-int erod_get_fused_config(const erod_batch* b, int* lanes_per_tile, int* steps_per_launch);
+/* Tile / launch shape of the persistent kernels: a workgroup owns whole rods
+ * totalling at most lanes_per_tile elements (64 ... 1024), each thread holding
+ * elements_per_thread of them (1, 2 or 4); 0 leaves a value free.
+ * steps_per_launch: steps a workgroup advances with its rods' state held on
+ * chip (0 = 100).  The rods of a batch share one shape. */
+int erod_set_fused_config(erod_batch* b, int lanes_per_tile, int elements_per_thread, int steps_per_launch);
+int erod_get_fused_config(const erod_batch* b, int* lanes_per_tile, int* elements_per_thread, int* steps_per_launch);
+/* Times every feasible tile shape on `trial_steps` steps from a snapshot of
+ * the state (restored afterwards) and keeps the fastest; reports its ms/step.
+ * No counterpart in the reference (its blocks are sized by a fixed policy,
+ * elastica/Simulator/Configuration/ModuleConfiguration/BlockingPolicy.hpp:291-321). */
+int erod_autotune_fused(erod_batch* b, int precision, int trial_steps, double* best_ms_per_step);
 /* n position-Verlet steps with the fused persistent kernels at the given
  * precision.  Keeps x, v, Q, w (and the rod clocks) current; derived fields
  * are refreshed on request by erod_compute_internal_dynamics. */

@@ -23,13 +23,16 @@
 //   * the PyElastica golden vectors of FullRodTest (0/1/2/3/5 steps, 1 and 2
 //     rods, damped and undamped, one force evaluation from the straight-rod
 //     initializer) at the reference's own tolerances;
-//   * closed forms: two-point difference / quadrature / average KATs,
-//     InvRotateDivide on rotate_via_axis_angle directors (kappa = (pi/5)(1,0,1)),
-//     the decimal Rodrigues matrices of Test_SO3PrimitiveAddAssign, and an
-//     independent matrix log / exp (scipy Rotation) on random directors;
-//   * physics: position Verlet on a harmonic oscillator, telescoping sum of
-//     undamped internal forces, zero loads on a straight unstressed rod,
-//     axial stretching tip displacement F L / (E A - F), Timoshenko deflection;
+//   * closed forms: two-point difference and quadrature against their
+//     zero-padded / trapezoid definitions, InvRotateDivide on
+//     rotate_via_axis_angle directors (kappa = (pi/5)(1,0,1)), the decimal
+//     Rodrigues matrices of Test_SO3PrimitiveAddAssign, and an independent
+//     matrix log / exp (scipy Rotation) on random directors;
+//   * physics: telescoping sum of undamped internal forces, zero loads on a
+//     straight unstressed rod, axial stretching tip displacement
+//     F L / (E A - F), first-order convergence of the Timoshenko cantilever
+//     deflection, exact free fall under gravity, a resting contact carrying
+//     the weight, sliding friction decelerating at mu g (damped closed form);
 //   * a rod stepped inside a multi-rod block equals the same rod alone, bitwise.
 #pragma once
 

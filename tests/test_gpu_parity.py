@@ -5,9 +5,13 @@ Tolerances
   * op-by-op FP64 path: the reference's golden tolerances (4e-13 .. 4e-12);
     against the oracle <= 1e-12 relative (same operation order, no FMA; only
     the device acos/sin/cos differ from glibc by an ulp).
-  * fused FP64 path: <= 1e-10 relative after hundreds of steps (FMA
-    contraction and the summation order of the torque terms differ).
+  * fused FP64 path: <= 1e-10 relative after hundreds of steps (it is built
+    without FMA contraction and keeps the reference's order of evaluation, so
+    on these batches it equals the op-by-op path bitwise; the bound leaves
+    room for the device math library).
   * FP32-bulk path: position error <= 1e-6 relative to the position scale.
+  * both fused paths: bitwise independent of the tile shape (lanes per tile,
+    elements per thread, steps per launch) and of the other rods in the batch.
 """
 import numpy as np
 import pytest
@@ -110,28 +114,49 @@ def test_fused_fp32_bulk_error_vs_oracle(ragged):
     assert err["Director"] <= 1e-5, err
 
 
-@pytest.mark.parametrize("lanes", [256, 512, 1024])
-def test_fused_is_independent_of_tiling(lanes):
-    """Per-rod arithmetic does not depend on which tile or lane a rod lands in."""
-    wl = random_workload(30, 2, 250, seed=11, steps=50)
+SHAPES = [(256, 1), (512, 1), (512, 2), (1024, 2), (512, 4), (1024, 4)]
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32_BULK])
+@pytest.mark.parametrize("lanes,epl", SHAPES)
+def test_fused_is_independent_of_tiling(lanes, epl, precision):
+    """Per-rod arithmetic does not depend on the tile, the thread or the
+    position within a thread a rod lands in."""
+    wl = random_workload(30, 4, 250, seed=11, steps=50)
     ref = device_block(wl)
-    ref.set_fused_config(1024, 25)
-    ref.step_fused(wl.steps, FP64)
+    ref.set_fused_config(256, 1, 25)
+    ref.step_fused(wl.steps, precision)
     blk = device_block(wl)
-    blk.set_fused_config(lanes, 10)
-    blk.step_fused(wl.steps, FP64)
+    blk.set_fused_config(lanes, epl, 10)
+    blk.step_fused(wl.steps, precision)
+    assert blk.fused_config()[:2] == (lanes, epl)
     for r in range(len(wl.n_elems)):
         for t in STATE:
-            assert np.array_equal(blk.get_rod(t, r), ref.get_rod(t, r)), (lanes, r, t)
+            assert np.array_equal(blk.get_rod(t, r), ref.get_rod(t, r)), (lanes, epl, r, t)
+
+
+def test_autotune_keeps_state_and_picks_a_shape():
+    wl = random_workload(25, 4, 200, seed=31, steps=30)
+    a = device_block(wl)
+    ms = a.autotune(FP32_BULK, 5)
+    assert ms > 0 and a.fused_config()[0] in (256, 512, 1024)
+    for r in range(len(wl.n_elems)):
+        assert np.array_equal(a.get_rod("Position", r), wl.states[r]["Position"])
+    a.step_fused(wl.steps, FP32_BULK)
+    b = device_block(wl)
+    b.step_fused(wl.steps, FP32_BULK)
+    for r in range(len(wl.n_elems)):
+        for t in STATE:
+            assert np.array_equal(a.get_rod(t, r), b.get_rod(t, r)), (r, t)
 
 
 def test_fused_graph_replay_equals_direct():
     wl = random_workload(20, 2, 120, seed=13, steps=60)
     a = device_block(wl)
-    a.set_fused_config(0, 20)
+    a.set_fused_config(0, 0, 20)
     a.step_fused(60, FP32_BULK)
     b = device_block(wl)
-    b.set_fused_config(0, 20)
+    b.set_fused_config(0, 0, 20)
     b.step_fused_graph(60, 20, FP32_BULK)
     for r in range(len(wl.n_elems)):
         for t in STATE:

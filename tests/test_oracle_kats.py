@@ -203,3 +203,59 @@ def test_timoshenko_cantilever_deflection_converges():
     e20, _ = _timoshenko_error(20)
     assert e20 <= 0.06 * ymax
     assert 1.8 <= e10 / e20 <= 2.2, (e10, e20)
+
+
+# --- environment hooks (gravity, plane contact, friction) ----------------------------
+
+def test_free_fall_is_exact_under_position_verlet():
+    """A straight unstressed rod under gravity translates rigidly; position
+    Verlet integrates a constant acceleration exactly: z(T) = z0 - g T^2 / 2."""
+    s = pyoracle.System([12], damped=False)
+    s.init_straight_rod(0, 0.02, 1.0, [0, 0, 5.0], [1, 0, 0], [0, 0, 1], 1e3, 1e6, 4e5, 0.0, 0.0)
+    s.set_gravity([0.0, 0.0, -9.81])
+    dt, n = 1e-4, 2000
+    s.step(dt, n)
+    T = dt * n
+    z = s.rod_field(0, "Position")[2]
+    assert np.allclose(z, 5.0 - 0.5 * 9.81 * T * T, rtol=0, atol=1e-12)
+    assert np.allclose(s.rod_field(0, "Velocity")[2], -9.81 * T, rtol=1e-12)
+
+
+def _resting_rod(vx=0.0, steps=20000):
+    n, r, L, rho = 10, 0.01, 0.5, 1e3
+    s = pyoracle.System([n])
+    s.init_straight_rod(0, r, L, [0, 0, r], [1, 0, 0], [0, 0, 1], rho, 1e6, 4e5, 1.0, 1.0)
+    s.set_gravity([0.0, 0.0, -9.81])
+    s.set_plane_contact(1e3, 10.0, 0.3, 1e4, 0.0, r)
+    if vx:
+        s.rod_field(0, "Velocity")[0] = vx
+    m = s.rod_field(0, "Mass")[0].sum()
+    return s, m, r
+
+
+def test_resting_contact_carries_the_weight():
+    """At rest on the plane z = 0 (contact offset = radius) the spring forces
+    k * delta of all nodes add up to the total weight."""
+    s, m, r = _resting_rod()
+    s.step(2e-5, 50000)
+    z = s.rod_field(0, "Position")[2]
+    delta = np.maximum(0.0, r - z)
+    assert abs(1e3 * delta.sum() - m * 9.81) <= 1e-6 * m * 9.81
+    assert np.abs(s.rod_field(0, "Velocity")).max() < 1e-4
+
+
+def test_sliding_friction_decelerates_at_mu_g():
+    """A rod sliding along x with saturated Coulomb friction (ct |v_t| > mu f_N,
+    LinearSpringDashpot.cpp:63-67) and the rod's own force damping -m gamma v
+    (ExplicitDampingAdapterFacade.hpp:193-197): dv/dt = -gamma v - mu g, so
+    v(T) = (v0 + mu g / gamma) exp(-gamma T) - mu g / gamma."""
+    s, m, r = _resting_rod()
+    s.step(2e-5, 50000)  # settle on the plane first
+    s.rod_field(0, "Velocity")[0] = 1.0
+    T = 0.2
+    s.step(2e-5, int(T / 2e-5))
+    v = s.rod_field(0, "Velocity")[0]
+    mass = s.rod_field(0, "Mass")[0]
+    vcm = (v * mass).sum() / mass.sum()
+    a = 0.3 * 9.81 / 1.0
+    assert abs(vcm - ((1.0 + a) * np.exp(-1.0 * T) - a)) <= 1e-3
