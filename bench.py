@@ -7,9 +7,15 @@ fused FP64 step, the plain (op-by-op) FP64 step and the CPU oracle.
 Prints ONE JSON line.  `value` is the FP32-bulk element-steps/s with the state
 resident in HBM (element-steps = sum of rod element counts x steps; ghost
 slots are not counted).  The roofline object prices the dominant kernel
-(the fused FP32-bulk step) by its algorithmic bytes B32(n) = 360 n + 72 per
-rod-step over the kernel's own duration, measured with HIP events on the
-batch stream.
+(the fused FP32-bulk step, `steps_per_launch` steps per launch) by its
+algorithmic bytes B32(n) = 360 n + 72 per rod-step times the steps of one
+launch, over the kernel's average launch duration, measured with HIP events
+on the batch stream around the timed region.  Because that kernel keeps the
+rod state on chip for all steps of a launch, its HBM traffic is far below the
+algorithmic bytes and the fraction can exceed 1; `roofline_streaming` prices
+the same kernel at one step per launch (state through HBM every step), and
+`traffic` holds the HBM bytes per launch measured with rocprofv3 counters
+(committed under profiles/, matched on the tile shape).
 """
 import argparse
 import json
@@ -41,6 +47,44 @@ def log(*a):
 def make_block(wl):
     from elasticapp_amd.workloads import build_block
     return build_block(wl)
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_fp32_ragged4096_pmc.md")
+
+
+def measured_traffic(lanes, epl, spl):
+    """HBM bytes per launch of the FP32-bulk kernel at this tile shape, from
+    the rocprofv3 FETCH_SIZE / WRITE_SIZE summary (lines of the form
+    `traffic lanes=L epl=P spl=S bytes_per_launch=B`), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                w = line.strip().strip("`").split()
+                if len(w) == 5 and w[0] == "traffic":
+                    kv = dict(x.split("=", 1) for x in w[1:])
+                    if (int(kv["lanes"]), int(kv["epl"]), int(kv["spl"])) == (lanes, epl, spl):
+                        return float(kv["bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        pass
+    return None
+
+
+def stream_copy_gbs(torch, nbytes=2 << 30, reps=10):
+    """Device-to-device copy rate (read + write bytes / s) of a 2 GiB buffer:
+    the measured HBM ceiling beside the 8 TB/s peak."""
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return gbs
 
 
 def timed(fn, sync):
@@ -89,6 +133,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--chunk", type=int, default=100, help="steps per captured hipGraph replay")
     ap.add_argument("--plain-steps", type=int, default=200, help="steps timed on the op-by-op FP64 path")
+    ap.add_argument("--stream-steps", type=int, default=200, help="steps timed at one step per launch")
+    ap.add_argument("--tile", default="", help="LANES,EPL: fixed tile shape instead of the autotuner's choice")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rods", type=int, default=4096)
     args = ap.parse_args()
@@ -112,11 +158,21 @@ def main():
     elems = int(n.sum())
     log("workload ragged_c2: %d rods, %d elements, dt %.3e" % (len(n), elems, wl.dt))
 
+    copy_gbs = stream_copy_gbs(torch)
+    log("device-to-device copy: %.0f GB/s" % copy_gbs)
+
     # ---- FP32-bulk, hipGraph chunks ----------------------------------------------------
     blk32 = make_block(wl)
-    blk32.set_fused_config(0, 0, args.chunk)
-    tune_ms = blk32.autotune(FP32_BULK, 20)
-    log("autotuned shape (lanes, elements/thread, steps/launch) = %s, %.4f ms/step" % (blk32.fused_config(), tune_ms))
+    if args.tile:
+        lanes, epl = (int(x) for x in args.tile.split(","))
+        blk32.set_fused_config(lanes, epl, args.chunk)
+        log("fixed shape (lanes, elements/thread, steps/launch) = %s" % (blk32.fused_config(),))
+    else:
+        blk32.set_fused_config(0, 0, args.chunk)
+        # one trial launch of `chunk` steps per tile shape
+        tune_ms = blk32.autotune(FP32_BULK, args.chunk)
+        log("autotuned shape (lanes, elements/thread, steps/launch) = %s, %.4f ms/step"
+            % (blk32.fused_config(), tune_ms))
 
     def sync32():
         blk32.synchronize()
@@ -126,16 +182,22 @@ def main():
     sync32()
     el32 = timed(lambda: blk32.step_fused_graph(args.steps, args.chunk, FP32_BULK), sync32)
     v32 = elems * args.steps / el32
-    log("fp32-bulk: %.3f s for %d steps -> %.3e element-steps/s" % (el32, args.steps, v32))
+    # dominant kernel: HIP events on the batch stream around the timed replays
+    kms, launches = blk32.last_kernel_ms()
+    log("fp32-bulk: %.3f s for %d steps -> %.3e element-steps/s; %d launches, %.4f ms/launch"
+        % (el32, args.steps, v32, launches, kms))
     flags = blk32.check_finite()
     assert not flags.any(), "non-finite FP32-bulk state in %d rods" % int(flags.sum())
 
     lanes, epl, spl = blk32.fused_config()
+    steps_per_launch = args.steps // launches
+    bytes_launch = float(b32(n).sum()) * steps_per_launch
+    achieved = bytes_launch / (kms * 1e-3) / 1e9
 
     # ---- fused FP64 over the same steps (error reference and FP64 rate) ----------------
     blk64 = make_block(wl)
     blk64.set_fused_config(0, 0, args.chunk)
-    blk64.autotune(FP64, 20)
+    blk64.autotune(FP64, args.chunk)
 
     def sync64():
         blk64.synchronize()
@@ -158,17 +220,19 @@ def main():
     log("fp32-bulk position error vs fp64 after %d steps: max_rod |dx|/L = %.3e" % (args.warmup + args.steps, pos_err))
     blk64.close()
 
-    # dominant kernel duration: launches of `chunk` steps timed with HIP events
-    # on the batch stream (after the error check: these steps advance blk32)
-    kms = []
-    for _ in range(5):
-        blk32.step_fused(args.chunk, FP32_BULK)
-        ms, _ = blk32.last_kernel_ms()
-        kms.append(ms)
-    kms = float(np.mean(kms))
-    bytes_launch = float(b32(n).sum()) * args.chunk
-    achieved = bytes_launch / (kms * 1e-3) / 1e9
+    # the same kernel and tile shape at one step per launch: the state goes
+    # through HBM every step (after the error check: these steps advance blk32)
+    blk32.set_fused_config(lanes, epl, 1)
+    blk32.step_fused(10, FP32_BULK)
+    blk32.step_fused(args.stream_steps, FP32_BULK)
+    kms_s, launches_s = blk32.last_kernel_ms()
+    bytes_s = float(b32(n).sum())
+    achieved_s = bytes_s / (kms_s * 1e-3) / 1e9
+    log("fp32-bulk at 1 step/launch: %.4f ms/launch -> %.0f GB/s algorithmic (%.3e element-steps/s)"
+        % (kms_s, achieved_s, elems / (kms_s * 1e-3)))
     blk32.close()
+    traffic = measured_traffic(lanes, epl, spl)
+    traffic_s = measured_traffic(lanes, epl, 1)
 
     # ---- plain FP64 (one kernel per reference operator) ----------------------------------
     blkp = make_block(wl)
@@ -202,9 +266,18 @@ def main():
                    "rods": int(len(n)), "elements": elems, "dt": wl.dt, "graph_chunk": args.chunk,
                    "tile_lanes": lanes, "elements_per_thread": epl, "steps_per_launch": spl, "device": dev_name},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_fused<float> (FP32-bulk)", "kernel_ms_per_launch": kms,
-                     "algorithmic_bytes_per_launch": bytes_launch},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_fused<float,%d,%d> (FP32-bulk)" % (epl, lanes // epl),
+                     "kernel_ms_per_launch": kms, "launches": launches, "steps_per_launch": steps_per_launch,
+                     "algorithmic_bytes_per_launch": bytes_launch,
+                     "note": "state held on chip across the steps of a launch: HBM traffic per launch is "
+                             "`traffic`, far below the algorithmic bytes, so frac can exceed 1"},
+        "roofline_streaming": {"bound": "hbm", "achieved": achieved_s, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": achieved_s / HBM_PEAK_GBS, "traffic": traffic_s,
+                               "kernel_ms_per_launch": kms_s, "launches": launches_s, "steps_per_launch": 1,
+                               "algorithmic_bytes_per_launch": bytes_s,
+                               "element_steps_per_s": elems / (kms_s * 1e-3)},
+        "stream_copy_gbs": copy_gbs,
         "cpu_baseline": cpu,
         "fp64_fused": {"value": v64, "ms_per_step": el64 * 1e3 / args.steps},
         "fp64_plain": {"value": vp, "steps": args.plain_steps},

@@ -453,13 +453,14 @@ int erod_step_fused(erod_batch* b, int precision, uint64_t n) {
   const uint64_t spl = b->plan.steps_per_launch > 0 ? uint64_t(b->plan.steps_per_launch) : 100;
   uint64_t done = 0;
   b->last_launches = 0;
+  EROD_HIP(hipEventRecord(b->ev0, b->stream));
   while (done < n) {
     const uint64_t s = std::min<uint64_t>(spl, n - done);
-    const bool timed = done == 0;
-    if ((rc = fused_launch(b, precision, static_cast<int>(s), timed))) return rc;
+    if ((rc = fused_launch(b, precision, static_cast<int>(s), false))) return rc;
     ++b->last_launches;
     done += s;
   }
+  EROD_HIP(hipEventRecord(b->ev1, b->stream));
   return EROD_OK;
 }
 
@@ -478,12 +479,13 @@ int erod_step_fused_graph(erod_batch* b, int precision, uint64_t n, uint64_t chu
     }
     hipGraph_t g = nullptr;
     EROD_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    uint64_t done = 0;
+    uint64_t done = 0, launches = 0;
     int lrc = EROD_OK;
     while (done < chunk && lrc == EROD_OK) {
       const uint64_t s = std::min<uint64_t>(spl, chunk - done);
       lrc = fused_launch(b, precision, static_cast<int>(s), false);
       done += s;
+      ++launches;
     }
     hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (lrc) return lrc;
@@ -492,8 +494,12 @@ int erod_step_fused_graph(erod_batch* b, int precision, uint64_t n, uint64_t chu
     hipGraphDestroy(g);
     b->graph_precision = precision;
     b->graph_chunk = chunk;
+    b->graph_launches = launches;
   }
+  EROD_HIP(hipEventRecord(b->ev0, b->stream));
   for (uint64_t i = 0; i < n / chunk; ++i) EROD_HIP(hipGraphLaunch(b->graph_exec, b->stream));
+  EROD_HIP(hipEventRecord(b->ev1, b->stream));
+  b->last_launches = (n / chunk) * b->graph_launches;
   return EROD_OK;
 }
 
@@ -504,10 +510,14 @@ int erod_reorthonormalize(erod_batch* b) {
 
 int erod_last_kernel_ms(const erod_batch* b, double* ms, uint64_t* launches) {
   if (check_batch(b)) return EROD_ERR_INVALID;
+  if (b->last_launches == 0) {
+    set_error("erod_last_kernel_ms: no fused call has run on this batch");
+    return EROD_ERR_INVALID;
+  }
   float f = 0.0f;
   EROD_HIP(hipEventSynchronize(b->ev1));
   EROD_HIP(hipEventElapsedTime(&f, b->ev0, b->ev1));
-  if (ms) *ms = f;
+  if (ms) *ms = f / static_cast<double>(b->last_launches);
   if (launches) *launches = b->last_launches;
   return EROD_OK;
 }

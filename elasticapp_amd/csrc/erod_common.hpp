@@ -111,6 +111,7 @@ struct erod_batch {
   erod::FusedPlan plan;
   // graph cache
   hipGraphExec_t graph_exec = nullptr;
+  uint64_t graph_launches = 0;  // kernel launches in one replay of graph_exec
   int graph_precision = -1;
   uint64_t graph_chunk = 0;
   // timing of the dominant kernel

@@ -277,6 +277,7 @@ namespace erod {
       rc = EROD_ERR_UNSUPPORTED;
     }
     if (best_ms) *best_ms = best_t / trial_steps;
+    b->last_launches = 0;  // the trial launches are not a fused call's timing
     return rc;
   }
 

@@ -161,8 +161,11 @@ int erod_step_fused(erod_batch* b, int precision, uint64_t n);
 int erod_step_fused_graph(erod_batch* b, int precision, uint64_t n, uint64_t chunk);
 /* Re-orthonormalise every director (one Newton-Schulz polar step, FP64). */
 int erod_reorthonormalize(erod_batch* b);
-/* Average duration (ms) of the dominant kernel over the last fused call,
- * measured with HIP events on the batch stream. */
+/* Average duration (ms) of one launch of the fused step kernel over the last
+ * erod_step_fused / erod_step_fused_graph call: HIP events recorded on the
+ * batch stream before the first and after the last launch, divided by the
+ * number of launches (reported in *launches).  The gaps between back-to-back
+ * launches are included. */
 int erod_last_kernel_ms(const erod_batch* b, double* ms, uint64_t* launches);
 
 /* ---- synchronisation and errors ------------------------------------------ */
