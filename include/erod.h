@@ -113,7 +113,8 @@ int erod_dynamic_update(erod_batch* b, double dt);
  * `index`, director of element `index`), rate constraint after the dynamic
  * step (ConstrainPrimitives / ConstrainRates,
  * elastica/Simulator/Interface/Constraints/ExecutionPolicy.hpp:591-593).
- * Only index 0 is supported by the fused path. */
+ * Only index 0 (the rod's first node and element, as in the reference's
+ * examples) is supported; other indices return EROD_ERR_UNSUPPORTED. */
 int erod_add_clamp(erod_batch* b, uint64_t rod, uint64_t index, const double x[3], const double Q[9]);
 /* ExternalLoads(:, last node) = F * ramp(t) before the dynamic step
  * (examples/timoshenko/TimoshenkoCaseStudy.cpp:128-141; ramps

@@ -26,10 +26,19 @@ VORONOI_FIELDS = ("ReferenceVoronoiLength", "VoronoiLength", "VoronoiDilatation"
                   "InternalCouple", "_DummyVoronoiVector")
 
 
-def _lib():
+def use_library(path=None):
+    """Load the oracle from `path` (default: oracle/liboracle.so) for every
+    System created afterwards; the mutation tests load deliberately broken
+    builds this way."""
+    global _LIB
+    _LIB = None
+    return _lib(path)
+
+
+def _lib(path=None):
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        path = path or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError("oracle/liboracle.so is not built: run `make -C oracle`")
         L = ctypes.CDLL(path)
@@ -87,7 +96,7 @@ class System:
     elements (0: one block per rod)."""
 
     def __init__(self, n_elems, max_block_elems=1024, damped=True):
-        L = _lib()
+        L = self._L = _lib()  # a System keeps the library it was created with
         self.n_elems = [int(n) for n in n_elems]
         arr = (ctypes.c_uint64 * len(self.n_elems))(*self.n_elems)
         self._h = L.oracle_system_create(arr, len(self.n_elems), int(max_block_elems), 1 if damped else 0)
@@ -96,7 +105,7 @@ class System:
 
     def close(self):
         if self._h:
-            _lib().oracle_system_destroy(self._h)
+            self._L.oracle_system_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -107,16 +116,16 @@ class System:
 
     @property
     def n_blocks(self):
-        return int(_lib().oracle_system_n_blocks(self._h))
+        return int(self._L.oracle_system_n_blocks(self._h))
 
     def location(self, rod):
         b, s = ctypes.c_uint64(), ctypes.c_uint64()
-        _lib().oracle_rod_location(self._h, rod, ctypes.byref(b), ctypes.byref(s))
+        self._L.oracle_rod_location(self._h, rod, ctypes.byref(b), ctypes.byref(s))
         return b.value, s.value
 
     def block_field(self, block, name):
         nc, ns = ctypes.c_uint64(), ctypes.c_uint64()
-        p = _lib().oracle_block_field(self._h, block, name.encode(), ctypes.byref(nc), ctypes.byref(ns))
+        p = self._L.oracle_block_field(self._h, block, name.encode(), ctypes.byref(nc), ctypes.byref(ns))
         if not p:
             raise KeyError(name)
         return np.ctypeslib.as_array(p, shape=(nc.value, ns.value))
@@ -129,57 +138,57 @@ class System:
 
     def fill_ghosts(self):
         for b in range(self.n_blocks):
-            _lib().oracle_block_fill_ghosts(self._h, b)
+            self._L.oracle_block_fill_ghosts(self._h, b)
 
     def init_straight_rod(self, rod, radius, length, origin, direction, normal,
                           density, youngs, shear, force_damping, torque_damping):
         p = np.array([radius, length, *origin, *direction, *normal, density, youngs, shear,
                       force_damping, torque_damping], dtype=np.float64)
-        _lib().oracle_initialize_straight_rod(self._h, rod, _dptr(p))
+        self._L.oracle_initialize_straight_rod(self._h, rod, _dptr(p))
 
     def compute_internal_forces(self):
-        _lib().oracle_compute_internal_forces(self._h)
+        self._L.oracle_compute_internal_forces(self._h)
 
     def compute_internal_torques(self):
-        _lib().oracle_compute_internal_torques(self._h)
+        self._L.oracle_compute_internal_torques(self._h)
 
     def update_acceleration(self):
-        _lib().oracle_update_acceleration(self._h)
+        self._L.oracle_update_acceleration(self._h)
 
     def kinematic_update(self, dt):
-        _lib().oracle_kinematic_update(self._h, dt)
+        self._L.oracle_kinematic_update(self._h, dt)
 
     def add_clamp(self, rod, x, Q, index=0):
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(3)
         Q = np.ascontiguousarray(Q, dtype=np.float64).reshape(9)
-        _lib().oracle_add_clamp(self._h, rod, index, _dptr(x), _dptr(Q))
+        self._L.oracle_add_clamp(self._h, rod, index, _dptr(x), _dptr(Q))
 
     def add_end_force(self, rod, F, ramp=0, start=0.0, interval=1.0):
         F = np.ascontiguousarray(F, dtype=np.float64).reshape(3)
-        _lib().oracle_add_end_force(self._h, rod, _dptr(F), int(ramp), float(start), float(interval))
+        self._L.oracle_add_end_force(self._h, rod, _dptr(F), int(ramp), float(start), float(interval))
 
     def set_gravity(self, g):
         g = np.ascontiguousarray(g, dtype=np.float64).reshape(3)
-        _lib().oracle_set_gravity(self._h, _dptr(g))
+        self._L.oracle_set_gravity(self._h, _dptr(g))
 
     def set_plane_contact(self, k, c, mu, ct, z0=0.0, offset=0.0):
-        _lib().oracle_set_plane_contact(self._h, k, c, mu, ct, z0, offset)
+        self._L.oracle_set_plane_contact(self._h, k, c, mu, ct, z0, offset)
 
     @property
     def time(self):
-        return _lib().oracle_time(self._h)
+        return self._L.oracle_time(self._h)
 
     @time.setter
     def time(self, t):
-        _lib().oracle_set_time(self._h, float(t))
+        self._L.oracle_set_time(self._h, float(t))
 
     def step(self, dt, n=1, threads=1):
-        _lib().oracle_step(self._h, float(dt), int(n), int(threads))
+        self._L.oracle_step(self._h, float(dt), int(n), int(threads))
 
     def step_per_block_dt(self, block_dt, n=1, threads=1):
         block_dt = np.ascontiguousarray(block_dt, dtype=np.float64)
         assert block_dt.size == self.n_blocks
-        rc = _lib().oracle_step_per_block_dt(self._h, _dptr(block_dt), int(n), int(threads))
+        rc = self._L.oracle_step_per_block_dt(self._h, _dptr(block_dt), int(n), int(threads))
         if rc != 0:
             raise ValueError("per-block dt stepping supports constant forces only")
 

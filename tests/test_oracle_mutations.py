@@ -1,0 +1,90 @@
+"""The golden vectors and KATs pin the oracle: a copy of the oracle with one
+plausible mistake planted (a dropped term, a wrong sign, a wrong index, a
+transposed operand, a wrong power) fails them, while an unchanged copy built
+the same way passes.
+
+Each mutation is one textual replacement in a copy of cosserat_oracle.hpp,
+compiled on the CPU into a temporary directory; nothing in oracle/ changes.
+"""
+import concurrent.futures
+import os
+import shutil
+import subprocess
+
+import pytest
+
+from oracle import pyoracle
+from tests import test_oracle_golden as golden
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE = os.path.join(os.path.dirname(HERE), "oracle")
+
+# (what the oracle says, the planted mistake, what kind of mistake)
+MUTATIONS = [
+    ("tau[c * E + k] += d1[c * E + k] * l[k];", "(void)0;", "dropped unsteady-dilatation torque"),
+    ("o[nin] = T(0.5) * a[nin - 1];", "o[nin] = T(0.25) * a[nin - 1];", "wrong quadrature end weight"),
+    ("- q[6 + j] * u[1]);", "+ q[6 + j] * u[1]);", "sign in the Rodrigues update"),
+    ("const T em = T(0.5) * (m[k] + m[k + 1]);", "const T em = T(0.5) * (m[k] + m[k]);", "index in the torque damping mass"),
+    ("D[i] = T(1.0) / (eps[i] * eps[i] * eps[i]);", "D[i] = T(1.0) / (eps[i] * eps[i]);", "power of the dilatation"),
+    ("cross(qt, nv, o);", "cross(nv, qt, o);", "transposed cross-product operands"),
+    ("d1[c * E + k] = d2[c * E + k] / e[k];", "d1[c * E + k] = d2[c * E + k] * e[k];", "J w / e inverted"),
+]
+
+FLAGS = ["-O1", "-ffp-contract=off", "-std=c++20", "-fPIC", "-pthread", "-shared"]
+
+
+def _build(tmp, i, mutation):
+    d = os.path.join(tmp, "m%d" % i)
+    os.makedirs(d)
+    for f in ("cosserat_oracle.hpp", "oracle_capi.cpp"):
+        shutil.copy(os.path.join(ORACLE, f), d)
+    if mutation is not None:
+        p = os.path.join(d, "cosserat_oracle.hpp")
+        src = open(p).read()
+        assert src.count(mutation[0]) >= 1, mutation[0]
+        open(p, "w").write(src.replace(mutation[0], mutation[1], 1))
+    so = os.path.join(d, "liboracle.so")
+    subprocess.run(["g++"] + FLAGS + ["-o", so, os.path.join(d, "oracle_capi.cpp")], check=True)
+    return so
+
+
+def _failing_golden_cases():
+    fails = 0
+    for p in golden._step_cases():
+        case, section = p.values
+        try:
+            golden.test_golden_steps(case, section)
+        except AssertionError:
+            fails += 1
+    try:
+        golden.test_golden_single_evaluation()
+    except AssertionError:
+        fails += 1
+    return fails
+
+
+@pytest.fixture(scope="module")
+def builds(tmp_path_factory):
+    tmp = str(tmp_path_factory.mktemp("oracle_mut"))
+    jobs = [None] + MUTATIONS
+    with concurrent.futures.ThreadPoolExecutor(max_workers=4) as ex:
+        libs = list(ex.map(lambda im: _build(tmp, im[0], im[1]), enumerate(jobs)))
+    yield libs
+    pyoracle.use_library(None)
+
+
+def test_unchanged_copy_passes(builds):
+    pyoracle.use_library(builds[0])
+    try:
+        assert _failing_golden_cases() == 0
+    finally:
+        pyoracle.use_library(None)
+
+
+@pytest.mark.parametrize("i", range(len(MUTATIONS)), ids=[m[2] for m in MUTATIONS])
+def test_planted_mistake_is_caught(builds, i):
+    pyoracle.use_library(builds[i + 1])
+    try:
+        assert _failing_golden_cases() > 0, MUTATIONS[i][2]
+    finally:
+        pyoracle.use_library(None)
