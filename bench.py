@@ -126,6 +126,54 @@ def cpu_baseline(wl, budget_s=12.0):
                       "%.1f s, %s" % (n_rods, elems, steps, el, cpu_model)}
 
 
+def c4_leg(args, torch):
+    """Rods on a plane under gravity with friction: 100,000 rods x 128
+    elements, FP32-bulk fused steps in hipGraph chunks, beside a few plain
+    FP64 steps."""
+    from elasticapp_amd import FP32_BULK
+    from elasticapp_amd.workloads import contact_c4
+    wl = contact_c4(steps=args.c4_steps)
+    n = np.array(wl.n_elems)
+    elems = int(n.sum())
+    blk = make_block(wl)
+
+    def sync():
+        blk.synchronize()
+        torch.cuda.synchronize()
+
+    blk.set_fused_config(0, 0, args.chunk)
+    blk.autotune(FP32_BULK, args.chunk)
+    blk.step_fused_graph(args.chunk, args.chunk, FP32_BULK)
+    sync()
+    el = timed(lambda: blk.step_fused_graph(args.c4_steps, args.chunk, FP32_BULK), sync)
+    kms, launches = blk.last_kernel_ms()
+    lanes, epl, spl = blk.fused_config()
+    flags = blk.check_finite()
+    assert not flags.any(), "non-finite state in %d contact rods" % int(flags.sum())
+    zmin = float(blk.get_field("Position")[2].min())
+    v = elems * args.c4_steps / el
+    bytes_launch = float(b32(n).sum()) * (args.c4_steps // launches)
+    achieved = bytes_launch / (kms * 1e-3) / 1e9
+    log("c4 contact: %d rods x %d, %.3f s for %d steps -> %.3e element-steps/s, %.0f GB/s algorithmic"
+        % (len(n), n[0], el, args.c4_steps, v, achieved))
+    # plain FP64 steps continue from the fused state on the same batch
+    blk.step(wl.dt, 2)
+    sync()
+    elp = timed(lambda: blk.step(wl.dt, args.c4_plain_steps), sync)
+    blk.close()
+    vp = elems * args.c4_plain_steps / elp
+    log("c4 contact plain fp64: %.3e element-steps/s" % vp)
+    return {"workload": "contact_c4: %d rods x %d elements, L 1 m, r 0.01 m, rho 1000, E 1e6, rest curvature "
+                        "N(0,0.5), gravity -9.81 z, plane z=0 k 1e3 c 10 mu 0.3 ct 1e2" % (len(n), n[0]),
+            "elements": elems, "dt": wl.dt, "steps": args.c4_steps, "value": v, "unit": "element-steps/s",
+            "ms_per_step": el * 1e3 / args.c4_steps, "tile_lanes": lanes, "elements_per_thread": epl,
+            "steps_per_launch": spl, "min_z": zmin,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "kernel_ms_per_launch": kms, "launches": launches,
+                         "algorithmic_bytes_per_launch": bytes_launch},
+            "fp64_plain": {"value": vp, "steps": args.c4_plain_steps}, "ratio_fp32bulk_vs_plain_fp64": v / vp}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +182,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=100, help="steps per captured hipGraph replay")
     ap.add_argument("--plain-steps", type=int, default=200, help="steps timed on the op-by-op FP64 path")
     ap.add_argument("--stream-steps", type=int, default=200, help="steps timed at one step per launch")
+    ap.add_argument("--c4-steps", type=int, default=2000, help="steps of the contact batch (0: skip it)")
+    ap.add_argument("--c4-plain-steps", type=int, default=20)
     ap.add_argument("--tile", default="", help="LANES,EPL: fixed tile shape instead of the autotuner's choice")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rods", type=int, default=4096)
@@ -142,8 +192,8 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path (rods shard by rod with no "
               "exchange; see the design notes)", file=sys.stderr)
         sys.exit(2)
-    if args.steps % args.chunk or args.warmup % args.chunk:
-        print("--steps and --warmup must be multiples of --chunk", file=sys.stderr)
+    if args.steps % args.chunk or args.warmup % args.chunk or args.c4_steps % args.chunk:
+        print("--steps, --warmup and --c4-steps must be multiples of --chunk", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -243,6 +293,8 @@ def main():
     log("plain fp64: %.3f s for %d steps -> %.3e element-steps/s" % (elp, args.plain_steps, vp))
     blkp.close()
 
+    c4 = c4_leg(args, torch) if args.c4_steps > 0 else None
+
     cpu = None if args.no_cpu else cpu_baseline(wl)
     if cpu:
         log("cpu oracle: %.3e element-steps/s on %d threads" % (cpu["value"], cpu["cores"]))
@@ -278,6 +330,7 @@ def main():
                                "algorithmic_bytes_per_launch": bytes_s,
                                "element_steps_per_s": elems / (kms_s * 1e-3)},
         "stream_copy_gbs": copy_gbs,
+        "c4_contact": c4,
         "cpu_baseline": cpu,
         "fp64_fused": {"value": v64, "ms_per_step": el64 * 1e3 / args.steps},
         "fp64_plain": {"value": vp, "steps": args.plain_steps},

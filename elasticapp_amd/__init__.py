@@ -378,7 +378,11 @@ def straight_rod_state(n, radius, length, origin, direction, normal, density, yo
 
 def assemble(block, rod_states):
     """Upload a list of per-rod state dicts (tag -> (ncomp, count) array) into
-    a DeviceBlock field by field, ghosts left at their ghost values."""
+    a DeviceBlock field by field, ghosts left at their ghost values.  A
+    container with a `fill_block(block)` method writes itself."""
+    if hasattr(rod_states, "fill_block"):
+        rod_states.fill_block(block)
+        return
     tags = set()
     for st in rod_states:
         tags.update(st.keys())

@@ -103,22 +103,82 @@ def ragged_c2(n_rods=4096, steps=10000, seed=2002, n_min=16, n_max=1024):
                     info=dict(rod_dt_min=float(dts.min()), rod_dt_max=float(dts.max())))
 
 
+class UniformRods:
+    """The per-rod states of equal rods built from one base rod plus a
+    per-rod position offset and per-rod rest curvature, made on demand.
+
+    Indexing gives the same per-rod dicts a list of states would (for the
+    oracle at test sizes); `fill_block` writes every rod into a DeviceBlock
+    with one whole-field upload per tag (for 1e5 rods)."""
+
+    def __init__(self, base, offsets, kappa0):
+        self.base = base
+        self.offsets = np.asarray(offsets, dtype=np.float64)    # (n_rods, 3)
+        self.kappa0 = np.asarray(kappa0, dtype=np.float64)      # (n_rods, 3, n - 1)
+
+    def __len__(self):
+        return len(self.offsets)
+
+    def _rod(self, i):
+        st = {k: v.copy() for k, v in self.base.items()}
+        st["Position"] = self.base["Position"] + self.offsets[i][:, None]
+        st["ReferenceCurvature"] = self.kappa0[i].copy()
+        return st
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return [self._rod(j) for j in range(*i.indices(len(self)))]
+        if i < 0:
+            i += len(self)
+        if not 0 <= i < len(self):
+            raise IndexError(i)
+        return self._rod(i)
+
+    def __iter__(self):
+        return (self._rod(i) for i in range(len(self)))
+
+    def rod_array(self, tag):
+        """(n_rods, ncomp, count) values of `tag` over all rods."""
+        if tag == "Position":
+            return self.base["Position"][None] + self.offsets[:, :, None]
+        if tag == "ReferenceCurvature":
+            return self.kappa0
+        b = self.base[tag]
+        return np.broadcast_to(b, (len(self),) + b.shape)
+
+    def fill_block(self, block):
+        # equal rods start every n + 2 slots: view each field as
+        # (ncomp, rods, n + 2) and write the rods' leading slots in one go
+        n = self.base["Director"].shape[1]
+        period = n + 2
+        rods = len(self)
+        starts = np.asarray(block.starts, dtype=np.int64)
+        if not np.array_equal(starts, period * np.arange(rods)):
+            raise ValueError("UniformRods.fill_block: block does not hold these rods in order")
+        for tag in sorted(self.base):
+            full = block.get_field(tag)
+            arr = self.rod_array(tag)
+            nc, count = arr.shape[1], arr.shape[2]
+            buf = np.empty((nc, rods * period))
+            buf[:, :full.shape[1]] = full
+            buf.reshape(nc, rods, period)[:, :, :count] = arr.transpose(1, 0, 2)
+            block.set_field(tag, buf[:, :full.shape[1]])
+
+
 def contact_c4(n_rods=100000, n=128, steps=2000, seed=4004, k=1e3, c=10.0, mu=0.3, ct=1e2):
-    """Rods of length 1 m and radius 0.01 m lying on z = radius (just touching
-    the plane z = 0 with the contact offset equal to the radius), random rest
-    curvature, gravity (0, 0, -9.81), per-node plane contact k = 1e3, c = 10,
-    mu = 0.3 and a tangential damping ct = 100 for the regularised friction."""
+    """Rods of length 1 m and radius 0.01 m (rho 1000, E 1e6, damping 0.1)
+    lying along x at z = radius, 5 cm apart in y, so that they touch the plane
+    z = 0 (contact surface z0 + offset with offset = radius); rest curvature
+    N(0, 0.5) per component, gravity (0, 0, -9.81), per-node plane contact
+    k = 1e3 N/m, c = 10 N s/m, mu = 0.3, tangential damping ct = 100 N s/m for
+    the regularised friction.  dt = 3.5e-5 s by the stable-dt rule."""
     rng = np.random.default_rng(seed)
     L, r, rho, E = 1.0, 0.01, 1000.0, 1e6
     G = E / (1.0 + 0.99)
     base = straight_rod_state(n, r, L, [0.0, 0.0, r], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0], rho, E, G, 0.1, 0.1)
-    states = []
-    for i in range(n_rods):
-        st = {k_: v.copy() for k_, v in base.items()}
-        st["Position"] = base["Position"].copy()
-        st["Position"][1] += 0.05 * i
-        st["ReferenceCurvature"] = rng.normal(0.0, 0.5, (3, n - 1))
-        states.append(st)
+    offsets = np.zeros((n_rods, 3))
+    offsets[:, 1] = 0.05 * np.arange(n_rods)
+    states = UniformRods(base, offsets, rng.normal(0.0, 0.5, (n_rods, 3, n - 1)))
     dt = stable_dt(L, n, E, rho)
     return Workload("contact_c4", [n] * n_rods, states, float(dt), steps, gravity=(0.0, 0.0, -9.81),
                     contact=(k, c, mu, ct, 0.0, r), info=dict(length=L, radius=r, density=rho, youngs=E))

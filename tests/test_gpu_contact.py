@@ -1,0 +1,66 @@
+"""Rods resting on a plane under gravity with friction (the 100,000 x 128
+contact batch and a 48-rod cut of it) on the GPU.
+
+  * the 48-rod batch against the oracle: op-by-op FP64 <= 1e-12, fused FP64
+    <= 1e-10, FP32-bulk positions <= 1e-6 relative;
+  * the full batch: every path stays finite, no node sinks deeper into the
+    plane than a bound well above the static deflection m g / k, and rods
+    stepped alone reproduce their in-batch state bitwise.
+"""
+import numpy as np
+import pytest
+
+from elasticapp_amd import FP32_BULK, FP64
+from elasticapp_amd.workloads import Workload, build_block, contact_c4
+from tests.gpu_helpers import STATE, oracle_system, rel_state_error
+
+pytestmark = pytest.mark.gpu
+
+
+def test_contact_cut_matches_oracle():
+    wl = contact_c4(n_rods=48, n=128, steps=300)
+    sysm = oracle_system(wl)
+    sysm.step(wl.dt, wl.steps)
+    a = build_block(wl)
+    a.step(wl.dt, wl.steps)
+    err = rel_state_error(a, sysm)
+    assert max(err.values()) <= 1e-12, err
+    b = build_block(wl)
+    b.step_fused(wl.steps, FP64)
+    err = rel_state_error(b, sysm)
+    assert max(err.values()) <= 1e-10, err
+    c = build_block(wl)
+    c.step_fused(wl.steps, FP32_BULK)
+    err = rel_state_error(c, sysm)
+    assert err["Position"] <= 1e-6, err
+    # the rods came down onto the plane and are held by it
+    z = np.stack([a.get_rod("Position", r)[2] for r in range(48)])
+    assert z.min() < 0.01 and z.min() > 0.01 - 1e-3
+
+
+STEPS = 200
+
+
+@pytest.fixture(scope="module")
+def c4():
+    return contact_c4(steps=STEPS)
+
+
+def test_full_contact_batch(c4):
+    blk = build_block(c4)
+    blk.step_fused_graph(STEPS, 100, FP32_BULK)
+    assert not blk.check_finite().any()
+    x = blk.get_field("Position")
+    # static deflection m g / k = (rho A L / n) g / k ~ 2.4e-5 m; allow 40x for the dynamics
+    r = c4.info["radius"]
+    m = c4.info["density"] * np.pi * r * r * c4.info["length"] / 128
+    sink = r - x[2, np.concatenate([np.arange(s, s + 129) for s in blk.starts[:2000]])].min()
+    assert sink < 40 * m * 9.81 / 1e3, sink
+    for rod in (0, 77777, 99999):
+        one = Workload("one", [128], [c4.states[rod]], c4.dt, STEPS, gravity=c4.gravity, contact=c4.contact)
+        b1 = build_block(one)
+        b1.step_fused_graph(STEPS, 100, FP32_BULK)
+        for t in STATE:
+            assert np.array_equal(b1.get_rod(t, 0), blk.get_rod(t, rod)), (rod, t)
+        b1.close()
+    blk.close()
