@@ -164,7 +164,7 @@ def c4_leg(args, torch):
     vp = elems * args.c4_plain_steps / elp
     log("c4 contact plain fp64: %.3e element-steps/s" % vp)
     return {"workload": "contact_c4: %d rods x %d elements, L 1 m, r 0.01 m, rho 1000, E 1e6, rest curvature "
-                        "N(0,0.5), gravity -9.81 z, plane z=0 k 1e3 c 10 mu 0.3 ct 1e2" % (len(n), n[0]),
+                        "N(0,0.5), gravity -9.81 z, plane z=0 k 1e3 c 10 mu 0.3 ct 10" % (len(n), n[0]),
             "elements": elems, "dt": wl.dt, "steps": args.c4_steps, "value": v, "unit": "element-steps/s",
             "ms_per_step": el * 1e3 / args.c4_steps, "tile_lanes": lanes, "elements_per_thread": epl,
             "steps_per_launch": spl, "min_z": zmin,
@@ -172,6 +172,61 @@ def c4_leg(args, torch):
                          "frac": achieved / HBM_PEAK_GBS, "kernel_ms_per_launch": kms, "launches": launches,
                          "algorithmic_bytes_per_launch": bytes_launch},
             "fp64_plain": {"value": vp, "steps": args.c4_plain_steps}, "ratio_fp32bulk_vs_plain_fp64": v / vp}
+
+
+def c3_leg(args, torch, capacity):
+    """Continuous batching: Poisson arrivals of requests (1-64 template rods of
+    32-512 elements, horizons of 1e3-1e5 steps) served from a pool of rod slots
+    by the FP32-bulk fused kernel, at an arrival rate chosen for ~70% of the
+    batched throughput `capacity`; latency = completion - arrival."""
+    from elasticapp_amd import FP32_BULK, Server
+    from elasticapp_amd.workloads import build_block, serving_library, serving_requests
+    wl, dts = serving_library()
+    lib = build_block(wl)
+    lib.set_rod_dt(dts)
+    n_lib = np.array(wl.n_elems)
+    reqs = serving_requests(args.c3_requests, len(n_lib))
+    work = np.array([n_lib[r].sum() * h for r, h in reqs], dtype=np.float64)
+    # mean work of the request distribution (sampled over many draws, not just this run's)
+    mean_work = float(np.mean([n_lib[r].sum() * h for r, h in serving_requests(20000, len(n_lib), seed=99)]))
+    lam = 0.7 * capacity / mean_work
+    rng = np.random.default_rng(3005)
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, len(reqs)))
+    classes = [(64, 8192), (128, 8192), (256, 8192), (512, 8192)]
+    srv = Server(lib, classes, precision=FP32_BULK, steps_per_tick=args.chunk, hbm_budget_bytes=200e9)
+    pool_bytes = srv.status()["pool_bytes"]
+    lat = np.full(len(reqs), np.nan)
+    nxt = done = ticks = 0
+    t0 = time.perf_counter()
+    while done < len(reqs):
+        now = time.perf_counter() - t0
+        while nxt < len(reqs) and arrivals[nxt] <= now:
+            srv.submit(nxt, reqs[nxt][0], reqs[nxt][1])
+            nxt += 1
+        st = srv.status()
+        if st["running"] == 0 and st["queued"] == 0 and nxt < len(reqs):
+            time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
+            continue
+        for rid in srv.tick():
+            srv.read(rid, 0, "Position")         # the result leaves the device
+            srv.release(rid)
+            lat[rid] = time.perf_counter() - t0 - arrivals[rid]
+            done += 1
+        ticks += 1
+    wall = time.perf_counter() - t0
+    srv.close()
+    lib.close()
+    thr = float(work.sum()) / wall
+    out = {"workload": "%d requests, U{1..64} rods of %d templates (n logU[32,512], materials as the ragged batch, "
+                       "own stable dt), horizons logU[1e3,1e5] steps, Poisson arrivals" % (len(reqs), len(n_lib)),
+           "lambda_per_s": lam, "target_utilization": 0.7, "capacity_element_steps_per_s": capacity,
+           "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+           "mean_s": float(np.mean(lat)), "max_s": float(np.max(lat)), "throughput": thr, "unit": "element-steps/s",
+           "utilization": thr / capacity, "wall_s": wall, "ticks": ticks, "steps_per_tick": args.chunk,
+           "pool_classes": classes, "pool_bytes": pool_bytes, "hbm_budget_bytes": 200e9}
+    log("c3 serving: lambda %.1f/s, p50 %.3f s, p99 %.3f s, %.3e element-steps/s (%.0f%% of capacity), %d ticks"
+        % (lam, out["p50_s"], out["p99_s"], thr, 100 * thr / capacity, ticks))
+    return out
 
 
 def main():
@@ -184,6 +239,7 @@ def main():
     ap.add_argument("--stream-steps", type=int, default=200, help="steps timed at one step per launch")
     ap.add_argument("--c4-steps", type=int, default=2000, help="steps of the contact batch (0: skip it)")
     ap.add_argument("--c4-plain-steps", type=int, default=20)
+    ap.add_argument("--c3-requests", type=int, default=400, help="requests of the serving leg (0: skip it)")
     ap.add_argument("--tile", default="", help="LANES,EPL: fixed tile shape instead of the autotuner's choice")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rods", type=int, default=4096)
@@ -294,6 +350,7 @@ def main():
     blkp.close()
 
     c4 = c4_leg(args, torch) if args.c4_steps > 0 else None
+    c3 = c3_leg(args, torch, v32) if args.c3_requests > 0 else None
 
     cpu = None if args.no_cpu else cpu_baseline(wl)
     if cpu:
@@ -331,6 +388,7 @@ def main():
                                "element_steps_per_s": elems / (kms_s * 1e-3)},
         "stream_copy_gbs": copy_gbs,
         "c4_contact": c4,
+        "c3_serving": c3,
         "cpu_baseline": cpu,
         "fp64_fused": {"value": v64, "ms_per_step": el64 * 1e3 / args.steps},
         "fp64_plain": {"value": vp, "steps": args.plain_steps},

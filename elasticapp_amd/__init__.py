@@ -17,7 +17,7 @@ import os
 
 import numpy as np
 
-__all__ = ["DeviceBlock", "FIELDS", "FP64", "FP32_BULK", "ErodError", "load_library", "library_path",
+__all__ = ["DeviceBlock", "Server", "FIELDS", "FP64", "FP32_BULK", "ErodError", "load_library", "library_path",
            "straight_rod_state", "assemble", "RAMP_NONE", "RAMP_LINEAR", "RAMP_SIN"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -98,6 +98,19 @@ def load_library():
         "erod_last_kernel_ms": (i32, [vp, P(dbl), P(u64)]),
         "erod_synchronize": (i32, [vp]),
         "erod_check_finite": (i32, [vp, P(ctypes.c_uint8)]),
+        "erod_set_rod_active_length": (i32, [vp, u64, u64]),
+        "erod_get_rod_active_length": (i32, [vp, u64, P(u64)]),
+        "erod_set_rod_step_budget": (i32, [vp, u64, u64]),
+        "erod_get_rod_step_budget": (i32, [vp, u64, P(u64)]),
+        "erod_copy_rods": (i32, [vp, P(u64), vp, P(u64), u64]),
+        "erod_server_create": (i32, [vp, P(u64), P(u64), i32, i32, i32, dbl, P(vp)]),
+        "erod_server_destroy": (i32, [vp]),
+        "erod_server_pool": (i32, [vp, P(vp)]),
+        "erod_server_submit": (i32, [vp, u64, P(u64), u64, u64]),
+        "erod_server_tick": (i32, [vp, P(u64), u64, P(u64)]),
+        "erod_server_read": (i32, [vp, u64, u64, i32, P(dbl)]),
+        "erod_server_release": (i32, [vp, u64]),
+        "erod_server_status": (i32, [vp, P(u64), P(u64), P(u64), P(u64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -120,6 +133,10 @@ def _check(rc):
 
 def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _u64p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
 
 
 class RodSlice:
@@ -185,7 +202,7 @@ class DeviceBlock:
 
     def rod_slots(self, tag, rod):
         _, _, st = FIELDS[tag]
-        n = self.n_elems[rod]
+        n = self.active_length(rod)
         return n + 1 if st == NODE else (n if st == ELEMENT else n - 1)
 
     # -- data movement --
@@ -306,6 +323,31 @@ class DeviceBlock:
     def synchronize(self):
         _check(_LIB.erod_synchronize(self._h))
 
+    # -- slots, budgets, rod copies --
+    def set_active_length(self, rod, n):
+        _check(_LIB.erod_set_rod_active_length(self._h, rod, n))
+
+    def active_length(self, rod):
+        n = ctypes.c_uint64()
+        _check(_LIB.erod_get_rod_active_length(self._h, rod, ctypes.byref(n)))
+        return n.value
+
+    def set_step_budget(self, rod, steps):
+        """Steps the fused path may still advance `rod` (None: unlimited)."""
+        _check(_LIB.erod_set_rod_step_budget(self._h, rod, (1 << 64) - 1 if steps is None else int(steps)))
+
+    def step_budget(self, rod):
+        n = ctypes.c_uint64()
+        _check(_LIB.erod_get_rod_step_budget(self._h, rod, ctypes.byref(n)))
+        return None if n.value == (1 << 64) - 1 else n.value
+
+    def copy_rods_from(self, src, dst_rods, src_rods):
+        """Device copy of rods `src_rods` of `src` into this batch's slots `dst_rods`."""
+        d = np.ascontiguousarray(dst_rods, dtype=np.uint64)
+        s_ = np.ascontiguousarray(src_rods, dtype=np.uint64)
+        assert d.shape == s_.shape
+        _check(_LIB.erod_copy_rods(self._h, _u64p(d), src._h, _u64p(s_), len(d)))
+
     def check_finite(self):
         out = np.zeros(self.n_rods, dtype=np.uint8)
         _check(_LIB.erod_check_finite(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
@@ -374,6 +416,92 @@ def straight_rod_state(n, radius, length, origin, direction, normal, density, yo
         "Velocity": np.zeros((3, n + 1)), "AngularVelocity": np.zeros((3, n)),
         "ReferenceCurvature": np.zeros((3, n - 1)), "ReferenceShearStretchStrain": np.zeros((3, n)),
     }
+
+
+class _PoolView(DeviceBlock):
+    """Non-owning DeviceBlock view of a server's pool batch."""
+
+    def __init__(self, handle, n_elems, damped):
+        self._h = handle
+        self.n_elems = list(n_elems)
+        self.damped = damped
+        self.starts = []
+        for r in range(len(self.n_elems)):
+            s_ = ctypes.c_uint64()
+            _check(_LIB.erod_rod_start(self._h, r, ctypes.byref(s_)))
+            self.starts.append(s_.value)
+
+    def close(self):
+        self._h = None
+
+
+class Server:
+    """Continuous batching of requests over a pool of rod slots (erod_server_*).
+
+    `classes` is a list of (slot capacity in elements, slot count); a request
+    is a list of rods of `library` (a DeviceBlock holding the template rods,
+    with their dt set) and a number of steps."""
+
+    def __init__(self, library, classes, precision=1, steps_per_tick=100, hbm_budget_bytes=200e9):
+        L = load_library()
+        caps = np.array([c for c, _ in classes], dtype=np.uint64)
+        cnts = np.array([k for _, k in classes], dtype=np.uint64)
+        h = ctypes.c_void_p()
+        _check(L.erod_server_create(library._h, _u64p(caps), _u64p(cnts), len(classes), precision, steps_per_tick,
+                                    float(hbm_budget_bytes), ctypes.byref(h)))
+        self._h = h
+        self.library = library
+        p = ctypes.c_void_p()
+        _check(L.erod_server_pool(self._h, ctypes.byref(p)))
+        n_elems = []
+        for cap, k in sorted(classes):
+            n_elems += [int(cap)] * int(k)
+        self.pool = _PoolView(p, n_elems, library.damped)
+        self._done = np.zeros(4096, dtype=np.uint64)
+        self._lib_n = [library.active_length(r) for r in range(len(library.n_elems))]
+        self._sizes = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _LIB.erod_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, rid, library_rods, steps):
+        r = np.ascontiguousarray(library_rods, dtype=np.uint64)
+        _check(_LIB.erod_server_submit(self._h, rid, _u64p(r), len(r), int(steps)))
+        self._sizes[rid] = [self._lib_n[int(x)] for x in r]
+
+    def tick(self):
+        """One tick; returns the ids of requests that finished."""
+        n = ctypes.c_uint64()
+        _check(_LIB.erod_server_tick(self._h, _u64p(self._done), len(self._done), ctypes.byref(n)))
+        return [int(x) for x in self._done[:n.value]]
+
+    def read(self, rid, rod, tag):
+        """`tag` of rod `rod` (index within the request) of a finished request."""
+        fid, nc, st = FIELDS[tag]
+        out = np.empty((nc, self._count(rid, rod, st)))
+        _check(_LIB.erod_server_read(self._h, rid, rod, fid, _dp(out)))
+        return out
+
+    def _count(self, rid, rod, st):
+        n = self._sizes[rid][rod]
+        return n + 1 if st == NODE else (n if st == ELEMENT else n - 1)
+
+    def release(self, rid):
+        _check(_LIB.erod_server_release(self._h, rid))
+        self._sizes.pop(rid, None)
+
+    def status(self):
+        q, r, e, b = (ctypes.c_uint64() for _ in range(4))
+        _check(_LIB.erod_server_status(self._h, ctypes.byref(q), ctypes.byref(r), ctypes.byref(e), ctypes.byref(b)))
+        return {"queued": q.value, "running": r.value, "active_elements": e.value, "pool_bytes": b.value}
 
 
 def assemble(block, rod_states):

@@ -27,6 +27,7 @@ SOURCES = {
     "erod_fused32.hip": ["-ffp-contract=off"],
     "erod_fused.hip": [],
     "erod_capi.hip": [],
+    "erod_serve.hip": [],
 }
 
 

@@ -30,9 +30,9 @@ namespace erod {
     EROD_HIP(hipMemcpyAsync(gt.data(), b->d_field[EROD_TORQUE_DAMPING_RATE], E * sizeof(double), hipMemcpyDeviceToHost, b->stream));
     EROD_HIP(hipStreamSynchronize(b->stream));
     for (uint64_t r = 0; r < b->n_rods; ++r) {
-      const uint64_t s = b->starts[r], n = b->n_elems[r];
-      double f = b->damped ? gf[s] : 0.0, t = b->damped ? gt[s] : 0.0;
-      if (b->damped) {
+      const uint64_t s = b->starts[r], n = b->n_active[r];
+      double f = b->damped && n ? gf[s] : 0.0, t = b->damped && n ? gt[s] : 0.0;
+      if (b->damped && n) {
         for (uint64_t i = 0; i <= n; ++i)
           if (gf[s + i] != f) { set_error("fused path needs a uniform ForceDampingRate per rod"); return EROD_ERR_UNSUPPORTED; }
         for (uint64_t i = 0; i < n; ++i)
@@ -69,6 +69,16 @@ namespace {
     if (f < 0 || f >= EROD_N_FIELDS) {
       set_error("field id out of range");
       return EROD_ERR_INVALID;
+    }
+    return EROD_OK;
+  }
+
+  // the op-by-op kernels run over the whole ghost-padded block
+  int check_plain(erod_batch* b) {
+    if (check_batch(b)) return EROD_ERR_INVALID;
+    if (!b->all_full) {
+      set_error("the op-by-op path needs every rod at its full length (see erod_set_rod_active_length)");
+      return EROD_ERR_UNSUPPORTED;
     }
     return EROD_OK;
   }
@@ -141,6 +151,10 @@ int erod_batch_create(const uint64_t* n_elems, uint64_t n_rods, int damped, erod
     if ((rc = hip_check(hipMemcpyAsync(b->d_ghost_nodes, b->ghost_nodes.data(), sizeof(uint64_t) * b->ghost_nodes.size(),
                                        hipMemcpyHostToDevice, b->stream), "copy ghosts"))) return fail(rc);
   }
+  if ((rc = hip_check(hipMalloc(&b->d_node_is_ghost, b->E + 1), "hipMalloc ghost flags"))) return fail(rc);
+  if ((rc = ops_mark_ghost_nodes(b))) return fail(rc);
+  b->n_active = b->n_elems;
+  b->steps_left.assign(n_rods, kNoBudget);
   b->rods.resize(n_rods);
   for (uint64_t r = 0; r < n_rods; ++r) {
     RodInfo& ri = b->rods[r];
@@ -166,6 +180,9 @@ int erod_batch_destroy(erod_batch* b) {
   if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
   for (int f = 0; f < EROD_N_FIELDS; ++f) hipFree(b->d_field[f]);
   hipFree(b->d_ghost_nodes);
+  hipFree(b->d_node_is_ghost);
+  hipFree(b->d_steps_left);
+  hipFree(b->d_copy_pairs);
   hipFree(b->d_rods);
   hipFree(b->d_time);
   hipFree(b->d_flags);
@@ -221,13 +238,13 @@ int erod_download_field(erod_batch* b, int field, double* host) {
 
 static uint64_t rod_count(const erod_batch* b, int field, uint64_t rod) {
   const int st = field_info(field).stagger;
-  const uint64_t n = b->n_elems[rod];
+  const uint64_t n = b->n_active[rod];
   return st == EROD_NODE ? n + 1 : (st == EROD_ELEMENT ? n : n - 1);
 }
 
 int erod_upload_rod(erod_batch* b, int field, uint64_t rod, const double* host) {
-  if (check_batch(b) || check_field(field) || !host || rod >= b->n_rods) {
-    set_error("erod_upload_rod: bad argument");
+  if (check_batch(b) || check_field(field) || !host || rod >= b->n_rods || b->n_active[rod] == 0) {
+    set_error("erod_upload_rod: bad argument (or an empty rod slot)");
     return EROD_ERR_INVALID;
   }
   const uint64_t cnt = rod_count(b, field, rod), slots = b->field_slots(field), s = b->starts[rod];
@@ -241,8 +258,8 @@ int erod_upload_rod(erod_batch* b, int field, uint64_t rod, const double* host) 
 }
 
 int erod_download_rod(erod_batch* b, int field, uint64_t rod, double* host) {
-  if (check_batch(b) || check_field(field) || !host || rod >= b->n_rods) {
-    set_error("erod_download_rod: bad argument");
+  if (check_batch(b) || check_field(field) || !host || rod >= b->n_rods || b->n_active[rod] == 0) {
+    set_error("erod_download_rod: bad argument (or an empty rod slot)");
     return EROD_ERR_INVALID;
   }
   const uint64_t cnt = rod_count(b, field, rod), slots = b->field_slots(field), s = b->starts[rod];
@@ -263,18 +280,18 @@ int erod_fill_ghosts(erod_batch* b) {
 }
 
 int erod_kinematic_update(erod_batch* b, double dt) {
-  if (check_batch(b)) return EROD_ERR_INVALID;
+  if (int rc = check_plain(b)) return rc;
   return ops_kinematic_update(b, dt);
 }
 
 int erod_compute_internal_dynamics(erod_batch* b, double time) {
   (void)time;
-  if (check_batch(b)) return EROD_ERR_INVALID;
+  if (int rc = check_plain(b)) return rc;
   return ops_compute_internal_dynamics(b);
 }
 
 int erod_dynamic_update(erod_batch* b, double dt) {
-  if (check_batch(b)) return EROD_ERR_INVALID;
+  if (int rc = check_plain(b)) return rc;
   return ops_dynamic_update(b, dt);
 }
 
@@ -337,26 +354,27 @@ int erod_set_plane_contact(erod_batch* b, double k, double c, double mu, double 
 
 int erod_apply_post_kinematic(erod_batch* b, double time) {
   (void)time;
-  if (check_batch(b)) return EROD_ERR_INVALID;
+  if (int rc = check_plain(b)) return rc;
   int rc = sync_rods(b);
   return rc ? rc : ops_apply_post_kinematic(b);
 }
 
 int erod_apply_pre_dynamic(erod_batch* b, double time) {
-  if (check_batch(b)) return EROD_ERR_INVALID;
+  if (int rc = check_plain(b)) return rc;
   int rc = sync_rods(b);
   return rc ? rc : ops_apply_pre_dynamic(b, time);
 }
 
 int erod_apply_post_dynamic(erod_batch* b, double time) {
   (void)time;
-  if (check_batch(b)) return EROD_ERR_INVALID;
+  if (int rc = check_plain(b)) return rc;
   int rc = sync_rods(b);
   return rc ? rc : ops_apply_post_dynamic(b);
 }
 
 int erod_step(erod_batch* b, double* time, double dt, uint64_t n) {
-  if (check_batch(b) || !time) return EROD_ERR_INVALID;
+  if (!time) return EROD_ERR_INVALID;
+  if (int rc = check_plain(b)) return rc;
   int rc = sync_rods(b);
   if (rc) return rc;
   double t = *time;
@@ -408,13 +426,120 @@ static int fused_common(erod_batch* b, int precision) {
     return EROD_ERR_INVALID;
   }
   for (uint64_t r = 0; r < b->n_rods; ++r)
-    if (!(b->rods[r].dt > 0.0)) {
+    if (b->n_active[r] && !(b->rods[r].dt > 0.0)) {
       set_error("fused path: set a positive dt for every rod with erod_set_rod_dt");
       return EROD_ERR_INVALID;
     }
   int rc = ensure_damping(b);
   if (rc) return rc;
+  if (b->budgets && b->budgets_dirty) {
+    EROD_HIP(hipMemcpyAsync(b->d_steps_left, b->steps_left.data(), sizeof(uint32_t) * b->n_rods, hipMemcpyHostToDevice,
+                            b->stream));
+    b->budgets_dirty = false;
+  }
   return fused_prepare(b, precision);
+}
+
+// The kernel spends min(budget, s) steps of each planned rod's budget per
+// launch of s steps; the host copy follows the same rule.  A rod that runs
+// out leaves the plan at the next call.
+static void spend_budgets(erod_batch* b, uint64_t s) {
+  if (!b->budgets) return;
+  bool emptied = false;
+  for (uint64_t r = 0; r < b->n_rods; ++r) {
+    uint32_t& l = b->steps_left[r];
+    if (!b->n_active[r] || l == kNoBudget || l == 0) continue;
+    l = l > s ? uint32_t(l - s) : 0u;
+    emptied |= l == 0;
+  }
+  if (emptied) b->plan.valid = false;
+}
+
+int erod_set_rod_active_length(erod_batch* b, uint64_t rod, uint64_t n) {
+  if (check_batch(b) || rod >= b->n_rods || n == 1 || n > b->n_elems[rod]) {
+    set_error("erod_set_rod_active_length: rod out of range, or n not 0 and not in [2, capacity]");
+    return EROD_ERR_INVALID;
+  }
+  b->n_active[rod] = n;
+  b->rods[rod].n = static_cast<uint32_t>(n);
+  b->rods_dirty = true;
+  b->damping_dirty = true;
+  b->all_full = b->n_active == b->n_elems;
+  b->plan.valid = false;
+  b->graph_precision = -1;
+  return EROD_OK;
+}
+
+int erod_get_rod_active_length(const erod_batch* b, uint64_t rod, uint64_t* n) {
+  if (check_batch(b) || rod >= b->n_rods || !n) return EROD_ERR_INVALID;
+  *n = b->n_active[rod];
+  return EROD_OK;
+}
+
+static int ensure_budget_buffer(erod_batch* b) {
+  if (b->d_steps_left) return EROD_OK;
+  EROD_HIP(hipMalloc(&b->d_steps_left, sizeof(uint32_t) * b->n_rods));
+  b->budgets_dirty = true;
+  return EROD_OK;
+}
+
+int erod_set_rod_step_budget(erod_batch* b, uint64_t rod, uint64_t steps) {
+  if (check_batch(b) || rod >= b->n_rods) return EROD_ERR_INVALID;
+  if (int rc = ensure_budget_buffer(b)) return rc;
+  const uint32_t v = steps >= kNoBudget ? kNoBudget : static_cast<uint32_t>(steps);
+  if ((v == 0) != (b->steps_left[rod] == 0)) {
+    b->plan.valid = false;
+    b->graph_precision = -1;
+  }
+  b->steps_left[rod] = v;
+  b->budgets = true;
+  b->budgets_dirty = true;
+  return EROD_OK;
+}
+
+int erod_get_rod_step_budget(const erod_batch* b, uint64_t rod, uint64_t* steps) {
+  if (check_batch(b) || rod >= b->n_rods || !steps) return EROD_ERR_INVALID;
+  *steps = b->steps_left[rod] == kNoBudget ? UINT64_MAX : b->steps_left[rod];
+  return EROD_OK;
+}
+
+int erod_copy_rods(erod_batch* dst, const uint64_t* dst_rods, erod_batch* src, const uint64_t* src_rods, uint64_t count) {
+  if (check_batch(dst) || check_batch(src) || (count && (!dst_rods || !src_rods)) || dst == src) {
+    set_error("erod_copy_rods: bad argument (two distinct batches needed)");
+    return EROD_ERR_INVALID;
+  }
+  if (dst->device != src->device || dst->damped != src->damped) {
+    set_error("erod_copy_rods: batches on different devices or of different damping models");
+    return EROD_ERR_INVALID;
+  }
+  std::vector<CopyPair> pairs(count);
+  for (uint64_t i = 0; i < count; ++i) {
+    const uint64_t d = dst_rods[i], r = src_rods[i];
+    if (d >= dst->n_rods || r >= src->n_rods || src->n_active[r] == 0 || src->n_active[r] > dst->n_elems[d]) {
+      set_error("erod_copy_rods: rod out of range, empty source, or source longer than the destination slot");
+      return EROD_ERR_INVALID;
+    }
+    pairs[i] = CopyPair{src->starts[r], dst->starts[d], static_cast<uint32_t>(src->n_active[r]), static_cast<uint32_t>(r),
+                        static_cast<uint32_t>(d), 0u};
+  }
+  // the per-rod damping rates travel in the rod records
+  if (int rc = ensure_damping(src)) return rc;
+  if (int rc = serve_copy_rods(dst, src, pairs)) return rc;
+  for (const CopyPair& p : pairs) {
+    RodInfo ri = src->rods[p.src_rod];
+    ri.start = dst->starts[p.dst_rod];
+    dst->rods[p.dst_rod] = ri;
+    dst->n_active[p.dst_rod] = p.n;
+    dst->steps_left[p.dst_rod] = kNoBudget;
+    dst->has_clamps |= (ri.flags & 1u) != 0;
+    dst->has_forces |= (ri.flags & 2u) != 0;
+  }
+  dst->rods_dirty = true;
+  dst->budgets_dirty = true;
+  dst->all_full = dst->n_active == dst->n_elems;
+  dst->plan.valid = false;
+  dst->graph_precision = -1;
+  return EROD_OK;
 }
 
 int erod_set_fused_config(erod_batch* b, int lanes_per_tile, int elements_per_thread, int steps_per_launch) {
@@ -457,6 +582,7 @@ int erod_step_fused(erod_batch* b, int precision, uint64_t n) {
   while (done < n) {
     const uint64_t s = std::min<uint64_t>(spl, n - done);
     if ((rc = fused_launch(b, precision, static_cast<int>(s), false))) return rc;
+    spend_budgets(b, s);
     ++b->last_launches;
     done += s;
   }
@@ -497,7 +623,10 @@ int erod_step_fused_graph(erod_batch* b, int precision, uint64_t n, uint64_t chu
     b->graph_launches = launches;
   }
   EROD_HIP(hipEventRecord(b->ev0, b->stream));
-  for (uint64_t i = 0; i < n / chunk; ++i) EROD_HIP(hipGraphLaunch(b->graph_exec, b->stream));
+  for (uint64_t i = 0; i < n / chunk; ++i) {
+    EROD_HIP(hipGraphLaunch(b->graph_exec, b->stream));
+    for (uint64_t d = 0; d < chunk; d += spl) spend_budgets(b, std::min<uint64_t>(spl, chunk - d));
+  }
   EROD_HIP(hipEventRecord(b->ev1, b->stream));
   b->last_launches = (n / chunk) * b->graph_launches;
   return EROD_OK;

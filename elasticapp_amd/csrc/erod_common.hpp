@@ -11,6 +11,15 @@
 #include "../../include/erod.h"
 
 namespace erod {
+  constexpr uint32_t kNoBudget = 0xFFFFFFFFu;
+  // one rod copied between batches: slots, active length, rod ids
+  struct CopyPair {
+    uint64_t src_start, dst_start;
+    uint32_t n, src_rod, dst_rod, pad;
+  };
+}
+
+namespace erod {
 
   struct FieldInfo {
     int ncomp;
@@ -82,6 +91,7 @@ namespace erod {
     uint32_t* d_tile_rods = nullptr;  // rod ids in tile order
     uint32_t* d_tile_lane0 = nullptr; // first lane of each rod within its tile
     Tile* d_tiles = nullptr;
+    uint64_t rod_capacity = 0;        // entries allocated in each of the three arrays
     bool valid = false;
   };
 
@@ -93,9 +103,19 @@ struct erod_batch {
   uint64_t n_rods = 0;
   uint64_t E = 0;  // element slots incl. ghosts
   bool damped = true;
-  std::vector<uint64_t> n_elems, starts, ghost_nodes;
+  std::vector<uint64_t> n_elems, starts, ghost_nodes;  // n_elems: the slot capacity of each rod
+  // active length of each rod (<= its capacity; 0 = empty slot).  Equal to
+  // n_elems unless erod_set_rod_active_length changed it; the op-by-op path
+  // needs every rod at full length.
+  std::vector<uint64_t> n_active;
+  bool all_full = true;
+  // per-rod step budgets of the fused path (kNoBudget = unlimited)
+  std::vector<uint32_t> steps_left;
+  uint32_t* d_steps_left = nullptr;
+  bool budgets = false, budgets_dirty = false;
   double* d_field[EROD_N_FIELDS] = {};
   uint64_t* d_ghost_nodes = nullptr;
+  uint8_t* d_node_is_ghost = nullptr;  // N flags: 1 on the ghost node between two rods
   // rods, hooks, clocks
   std::vector<erod::RodInfo> rods;
   erod::RodInfo* d_rods = nullptr;
@@ -120,6 +140,8 @@ struct erod_batch {
   uint64_t last_launches = 0;
   // scratch
   int* d_flags = nullptr;
+  erod::CopyPair* d_copy_pairs = nullptr;  // grow-only staging of erod_copy_rods
+  uint64_t copy_capacity = 0;
 
   uint64_t slots(int stagger) const { return stagger == EROD_NODE ? E + 1 : (stagger == EROD_ELEMENT ? E : E - 1); }
   uint64_t field_slots(int f) const { return slots(erod::field_info(f).stagger); }
@@ -156,6 +178,7 @@ namespace erod {
   int ops_compute_internal_dynamics(erod_batch* b);
   int ops_dynamic_update(erod_batch* b, double dt);
   int ops_fill_ghosts(erod_batch* b, int field);
+  int ops_mark_ghost_nodes(erod_batch* b);
   int ops_apply_post_kinematic(erod_batch* b);
   int ops_apply_pre_dynamic(erod_batch* b, double time);
   int ops_apply_post_dynamic(erod_batch* b);
@@ -166,6 +189,11 @@ namespace erod {
   int fused_reorthonormalize(erod_batch* b);
   int fused_check_finite(erod_batch* b, uint8_t* flags);
   int fused_autotune(erod_batch* b, int precision, int trial_steps, double* best_ms);
+  // offset (in floats) of a field in the FP32 constant copy, or ~0 if the field
+  // has none; valid while the copy is allocated and not dirty
+  uint64_t fused_const32_offset(const erod_batch* b, int field);
+  // device copy of whole rods between batches (admission of a request)
+  int serve_copy_rods(erod_batch* dst, const erod_batch* src, const std::vector<CopyPair>& pairs);
   int fused_shape_lanes(int shape);
   int fused_shape_epl(int shape);
   int sync_rods(erod_batch* b);

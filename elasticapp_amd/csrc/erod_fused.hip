@@ -23,6 +23,12 @@ namespace erod {
     return off;
   }
 
+  uint64_t fused_const32_offset(const erod_batch* b, int field) {
+    for (int i = 0; i < kNConst32; ++i)
+      if (kConst32Fields[i] == field) return const32_offset(b, i);
+    return ~uint64_t(0);
+  }
+
   using fused::Consts;
 
   namespace {
@@ -77,12 +83,20 @@ namespace erod {
 
   // Feasible shapes: capacity P * NT holds the longest rod, and every rod has
   // at least P elements (a thread then holds at most one rod end).
+  // rods the fused kernels advance: non-empty slots with steps left
+  static bool planned(const erod_batch* b, uint64_t r) {
+    return b->n_active[r] != 0 && (!b->budgets || b->steps_left[r] != 0);
+  }
+
   static bool shape_feasible(const erod_batch* b, int si) {
     uint64_t maxn = 0, minn = ~uint64_t(0);
-    for (auto n : b->n_elems) {
+    for (uint64_t r = 0; r < b->n_rods; ++r) {
+      if (!planned(b, r)) continue;
+      const uint64_t n = b->n_active[r];
       maxn = n > maxn ? n : maxn;
       minn = n < minn ? n : minn;
     }
+    if (maxn == 0) minn = maxn;  // nothing to run: every shape is feasible
     const fused::Shape sh = fused::kShapes[si];
     const int lanes = b->plan.lanes_per_tile, epl = b->plan.elements_per_thread;
     if (lanes && sh.P * sh.NT != lanes) return false;
@@ -103,13 +117,15 @@ namespace erod {
   // First-fit-decreasing bin packing of the rods into tiles of P * NT elements.
   static int plan_tiles(erod_batch* b, int si) {
     const uint64_t cap = uint64_t(fused::kShapes[si].P) * fused::kShapes[si].NT;
-    std::vector<uint32_t> order(b->n_rods);
-    for (uint32_t i = 0; i < b->n_rods; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return b->n_elems[a] > b->n_elems[c]; });
+    std::vector<uint32_t> order;
+    order.reserve(b->n_rods);
+    for (uint32_t i = 0; i < b->n_rods; ++i)
+      if (planned(b, i)) order.push_back(i);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return b->n_active[a] > b->n_active[c]; });
     std::vector<std::vector<uint32_t>> bins;
     std::vector<uint64_t> fill;
     for (uint32_t r : order) {
-      const uint64_t n = b->n_elems[r];
+      const uint64_t n = b->n_active[r];
       size_t bi = bins.size();
       // the open bins are the last few; older ones are (nearly) full
       for (size_t i = (bins.size() > 64 ? bins.size() - 64 : 0); i < bins.size(); ++i)
@@ -133,22 +149,33 @@ namespace erod {
       for (uint32_t r : bins[i]) {
         trods.push_back(r);
         tlane0.push_back(lane);
-        lane += static_cast<uint32_t>(b->n_elems[r]);
+        lane += static_cast<uint32_t>(b->n_active[r]);
       }
       tiles[i].lanes = lane;
       tiles[i].pad = 0;
     }
-    if (b->plan.d_tiles) EROD_HIP(hipFree(b->plan.d_tiles));
-    if (b->plan.d_tile_rods) EROD_HIP(hipFree(b->plan.d_tile_rods));
-    if (b->plan.d_tile_lane0) EROD_HIP(hipFree(b->plan.d_tile_lane0));
-    b->plan.d_tiles = nullptr;
-    b->plan.d_tile_rods = b->plan.d_tile_lane0 = nullptr;
-    EROD_HIP(hipMalloc(&b->plan.d_tiles, tiles.size() * sizeof(Tile)));
-    EROD_HIP(hipMalloc(&b->plan.d_tile_rods, trods.size() * sizeof(uint32_t)));
-    EROD_HIP(hipMalloc(&b->plan.d_tile_lane0, tlane0.size() * sizeof(uint32_t)));
-    EROD_HIP(hipMemcpyAsync(b->plan.d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, b->stream));
-    EROD_HIP(hipMemcpyAsync(b->plan.d_tile_rods, trods.data(), trods.size() * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
-    EROD_HIP(hipMemcpyAsync(b->plan.d_tile_lane0, tlane0.data(), tlane0.size() * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
+    // the plan arrays grow to the largest plan seen (a tile or a rod entry per
+    // rod at most) and are reused when the rod set changes
+    if (b->plan.rod_capacity < b->n_rods) {
+      EROD_HIP(hipStreamSynchronize(b->stream));
+      if (b->plan.d_tiles) EROD_HIP(hipFree(b->plan.d_tiles));
+      if (b->plan.d_tile_rods) EROD_HIP(hipFree(b->plan.d_tile_rods));
+      if (b->plan.d_tile_lane0) EROD_HIP(hipFree(b->plan.d_tile_lane0));
+      b->plan.d_tiles = nullptr;
+      b->plan.d_tile_rods = b->plan.d_tile_lane0 = nullptr;
+      b->plan.rod_capacity = 0;
+      EROD_HIP(hipMalloc(&b->plan.d_tiles, b->n_rods * sizeof(Tile)));
+      EROD_HIP(hipMalloc(&b->plan.d_tile_rods, b->n_rods * sizeof(uint32_t)));
+      EROD_HIP(hipMalloc(&b->plan.d_tile_lane0, b->n_rods * sizeof(uint32_t)));
+      b->plan.rod_capacity = b->n_rods;
+    }
+    if (!tiles.empty()) {
+      EROD_HIP(hipMemcpyAsync(b->plan.d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, b->stream));
+      EROD_HIP(hipMemcpyAsync(b->plan.d_tile_rods, trods.data(), trods.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              b->stream));
+      EROD_HIP(hipMemcpyAsync(b->plan.d_tile_lane0, tlane0.data(), tlane0.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              b->stream));
+    }
     EROD_HIP(hipStreamSynchronize(b->stream));
     b->plan.n_tiles = tiles.size();
     b->plan.shape = si;
@@ -247,6 +274,9 @@ namespace erod {
       return EROD_OK;
     };
     const int keep_lanes = b->plan.lanes_per_tile, keep_epl = b->plan.elements_per_thread;
+    // trial launches leave the step budgets alone
+    const bool keep_budgets = b->budgets;
+    b->budgets = false;
     b->plan.lanes_per_tile = 0;
     b->plan.elements_per_thread = 0;
     int best = -1;
@@ -267,6 +297,7 @@ namespace erod {
     }
     b->plan.lanes_per_tile = keep_lanes;
     b->plan.elements_per_thread = keep_epl;
+    b->budgets = keep_budgets;
     if (rc == EROD_OK && best >= 0) rc = plan_tiles(b, best);
     if (rc == EROD_OK) rc = restore();
     if (rc == EROD_OK) rc = hip_check(hipStreamSynchronize(b->stream), "autotune");

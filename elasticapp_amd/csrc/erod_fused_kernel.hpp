@@ -387,7 +387,7 @@ namespace erod {
                                                   double* __restrict__ W, Consts<R> C, const RodInfo* __restrict__ rods,
                                                   const Tile* __restrict__ tiles, const uint32_t* __restrict__ tile_rods,
                                                   const uint32_t* __restrict__ tile_lane0, double* __restrict__ rod_time,
-                                                  Env env, int nsteps, uint64_t E) {
+                                                  uint32_t* __restrict__ steps_left, Env env, int nsteps, uint64_t E) {
       constexpr bool kCache = sizeof(R) == 4;  // FP32 constants stay in registers for the launch
       const uint64_t N = E + 1;
       const Tile tile = tiles[blockIdx.x];
@@ -406,6 +406,7 @@ namespace erod {
       double x[P][3], v[P][3], q[P][9], w[P][3];
       double tm[P], dt[P];
       uint32_t rflags[P];
+      uint32_t lim[P];  // steps this launch may take for the element's rod (its step budget)
       R gfj[P], gtj[P];
       ElemConst<R> kc[kCache ? P : 1];
       R tail_m = R(0), tail_im = R(0);
@@ -430,6 +431,7 @@ namespace erod {
         slot[j] = ri.start + uint64_t(kk[j]);
         dt[j] = ri.dt;
         tm[j] = rod_time[r];
+        lim[j] = steps_left ? steps_left[r] : kNoBudget;
         rflags[j] = ri.flags;
         gfj[j] = static_cast<R>(ri.gamma_f);
         gtj[j] = static_cast<R>(ri.gamma_t);
@@ -462,10 +464,15 @@ namespace erod {
       }
 
       for (int step = 0; step < nsteps; ++step) {
+        // a rod whose budget is spent keeps its state (its element forces are
+        // still evaluated, since the tile moves in lockstep, but never applied)
+        bool live[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) live[j] = valid[j] && uint32_t(step) < lim[j];
         // ---- kinematic half step, time, post-kinematic clamp --------------------
 #pragma unroll
         for (int j = 0; j < P; ++j) {
-          if (!valid[j]) continue;
+          if (!live[j]) continue;
           const double ch = 0.5 * dt[j];
 #pragma unroll
           for (int c = 0; c < 3; ++c) x[j][c] = maddd<R>(ch, v[j][c], x[j][c]);
@@ -582,7 +589,7 @@ namespace erod {
         const int tp = tid > 0 ? tid - 1 : 0;
 #pragma unroll
         for (int j = 0; j < P; ++j) {
-          if (!valid[j]) continue;
+          if (!live[j]) continue;
           const int k = kk[j];
           const bool first = k == 0, last = k == nn[j] - 1;
           R cm[3] = {R(0), R(0), R(0)}, Pm[3] = {R(0), R(0), R(0)}, Rm[3] = {R(0), R(0), R(0)};
@@ -681,7 +688,7 @@ namespace erod {
         // ---- time, kinematic half step ------------------------------------------
 #pragma unroll
         for (int j = 0; j < P; ++j) {
-          if (!valid[j]) continue;
+          if (!live[j]) continue;
           tm[j] += 0.5 * dt[j];
           const double ch = 0.5 * dt[j];
 #pragma unroll
@@ -713,7 +720,10 @@ namespace erod {
             Vel[c * N + s + 1] = vt[c];
           }
         }
-        if (kk[j] == 0) rod_time[rodi[j]] = tm[j];
+        if (kk[j] == 0) {
+          rod_time[rodi[j]] = tm[j];
+          if (steps_left && lim[j] != kNoBudget) steps_left[rodi[j]] = lim[j] > uint32_t(nsteps) ? lim[j] - uint32_t(nsteps) : 0u;
+        }
       }
     }
 
@@ -723,7 +733,7 @@ namespace erod {
       hipLaunchKernelGGL((k_fused<R, P, NT>), dim3(static_cast<unsigned>(b->plan.n_tiles)), dim3(NT), 0, b->stream,
                          b->d_field[EROD_POSITION], b->d_field[EROD_VELOCITY], b->d_field[EROD_DIRECTOR],
                          b->d_field[EROD_ANGULAR_VELOCITY], C, b->d_rods, b->plan.d_tiles, b->plan.d_tile_rods,
-                         b->plan.d_tile_lane0, b->d_time, b->env, steps, b->E);
+                         b->plan.d_tile_lane0, b->d_time, b->budgets ? b->d_steps_left : nullptr, b->env, steps, b->E);
       return hip_check(hipGetLastError(), "k_fused");
     }
 

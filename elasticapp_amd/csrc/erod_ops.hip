@@ -341,15 +341,22 @@ namespace erod {
     }
 
     // gravity and per-node plane contact on every real node
+    __global__ void k_mark_ghost_nodes(uint8_t* __restrict__ flag, const uint64_t* __restrict__ ghost, uint64_t n) {
+      GRID_STRIDE(i, n) flag[ghost[i]] = 1;
+    }
+
+    // gravity on every node (the block-wide update of GravityField.hpp:239-246);
+    // plane contact on the rods' own nodes only
     __global__ void k_environment(double* __restrict__ fe, const double* __restrict__ m, const double* __restrict__ x,
-                                  const double* __restrict__ v, uint64_t E, Env env) {
+                                  const double* __restrict__ v, const uint8_t* __restrict__ is_ghost, uint64_t E,
+                                  Env env) {
       const uint64_t N = E + 1;
       GRID_STRIDE(i, N) {
         if (env.gravity)
           for (int c = 0; c < 3; ++c) fe[c * N + i] += env.g[c] * m[i];
         if (env.contact) {
           const double gap = x[2 * N + i] - env.z0 - env.offset;
-          if (gap < 0.0) {
+          if (gap < 0.0 && !is_ghost[i]) {
             const double delta = -gap;
             const double vn = -v[2 * N + i];
             const double fn = fmax(0.0, env.k * delta + env.c * vn);
@@ -370,6 +377,13 @@ namespace erod {
 
 #define L(kernel, n, ...) hipLaunchKernelGGL(kernel, dim3(grid_for(n)), dim3(kThreads), 0, b->stream, __VA_ARGS__)
 #define F(id) (b->d_field[id])
+
+  int ops_mark_ghost_nodes(erod_batch* b) {
+    EROD_HIP(hipMemsetAsync(b->d_node_is_ghost, 0, b->E + 1, b->stream));
+    if (!b->ghost_nodes.empty()) L(k_mark_ghost_nodes, b->ghost_nodes.size(), b->d_node_is_ghost, b->d_ghost_nodes,
+                                   static_cast<uint64_t>(b->ghost_nodes.size()));
+    return hip_check(hipGetLastError(), "mark ghost nodes");
+  }
 
   int ops_fill_ghosts(erod_batch* b, int field) {
     if (b->ghost_nodes.empty()) return EROD_OK;
@@ -490,7 +504,8 @@ namespace erod {
     const uint64_t N = b->E + 1;
     if (b->has_forces) L(k_end_force, b->n_rods, F(EROD_EXTERNAL_LOADS), b->d_rods, b->n_rods, b->E, time);
     if (b->env.gravity || b->env.contact)
-      L(k_environment, N, F(EROD_EXTERNAL_LOADS), F(EROD_MASS), F(EROD_POSITION), F(EROD_VELOCITY), b->E, b->env);
+      L(k_environment, N, F(EROD_EXTERNAL_LOADS), F(EROD_MASS), F(EROD_POSITION), F(EROD_VELOCITY), b->d_node_is_ghost,
+        b->E, b->env);
     return hip_check(hipGetLastError(), "pre_dynamic");
   }
 

@@ -10,6 +10,8 @@ random generator does not affect parity.
   ragged_c2          4096 rods, n log-uniform in [16, 1024], random materials,
                      constant tip forces
   contact_c4         rods lying above the plane z = 0 under gravity (contact)
+  serving_library    template rods for the serving workload, and
+  serving_requests   requests of 1-64 template rods with horizons of 1e3-1e5 steps
 
 Material conventions: G = E / (1 + 0.99) (a Poisson ratio of 0.99 as stated
 for the Timoshenko configuration); damping rates for forces and torques are
@@ -165,13 +167,15 @@ class UniformRods:
             block.set_field(tag, buf[:, :full.shape[1]])
 
 
-def contact_c4(n_rods=100000, n=128, steps=2000, seed=4004, k=1e3, c=10.0, mu=0.3, ct=1e2):
+def contact_c4(n_rods=100000, n=128, steps=2000, seed=4004, k=1e3, c=10.0, mu=0.3, ct=10.0):
     """Rods of length 1 m and radius 0.01 m (rho 1000, E 1e6, damping 0.1)
     lying along x at z = radius, 5 cm apart in y, so that they touch the plane
     z = 0 (contact surface z0 + offset with offset = radius); rest curvature
     N(0, 0.5) per component, gravity (0, 0, -9.81), per-node plane contact
-    k = 1e3 N/m, c = 10 N s/m, mu = 0.3, tangential damping ct = 100 N s/m for
-    the regularised friction.  dt = 3.5e-5 s by the stable-dt rule."""
+    k = 1e3 N/m, c = 10 N s/m, mu = 0.3, tangential damping ct = 10 N s/m for
+    the regularised friction.  dt = 3.5e-5 s by the stable-dt rule; with a
+    node mass of 2.45e-3 kg both dampers stay explicit-stable and non-stiff
+    (c dt / m = ct dt / m = 0.14)."""
     rng = np.random.default_rng(seed)
     L, r, rho, E = 1.0, 0.01, 1000.0, 1e6
     G = E / (1.0 + 0.99)
@@ -182,6 +186,42 @@ def contact_c4(n_rods=100000, n=128, steps=2000, seed=4004, k=1e3, c=10.0, mu=0.
     dt = stable_dt(L, n, E, rho)
     return Workload("contact_c4", [n] * n_rods, states, float(dt), steps, gravity=(0.0, 0.0, -9.81),
                     contact=(k, c, mu, ct, 0.0, r), info=dict(length=L, radius=r, density=rho, youngs=E))
+
+
+def serving_library(n_templates=2048, seed=3003, n_min=32, n_max=512):
+    """Template rods for the serving workload: n log-uniform in [32, 512],
+    materials and perturbations as in the ragged batch, a constant tip force
+    N(0, 1) N, and each rod's own stable dt (returned per rod)."""
+    rng = np.random.default_rng(seed)
+    n_elems = np.clip(np.rint(np.exp(rng.uniform(np.log(n_min), np.log(n_max), n_templates))).astype(int), n_min, n_max)
+    L = rng.uniform(0.5, 3.0, n_templates)
+    r = rng.uniform(0.005, 0.05, n_templates)
+    E = np.exp(rng.uniform(np.log(1e5), np.log(1e7), n_templates))
+    rho = rng.uniform(500.0, 5000.0, n_templates)
+    gam = rng.uniform(0.01, 0.5, n_templates)
+    F = rng.normal(0.0, 1.0, (n_templates, 3))
+    states, forces = [], []
+    for i in range(n_templates):
+        st = straight_rod_state(int(n_elems[i]), r[i], L[i], [0.0, 0.2 * i, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0],
+                                rho[i], E[i], E[i] / (1.0 + 0.99), gam[i], gam[i])
+        states.append(_randomize(st, rng))
+        forces.append((i, F[i].copy(), 0, 0.0, 1.0))
+    dts = stable_dt(L, n_elems, E, rho)
+    wl = Workload("serving_library", [int(x) for x in n_elems], states, float(dts.min()), 0, forces=forces)
+    return wl, dts
+
+
+def serving_requests(n_requests, n_templates, seed=3004, max_rods=64, min_steps=1000, max_steps=100000):
+    """Requests of U{1..64} template rods (drawn with replacement) and a
+    horizon log-uniform in [1e3, 1e5] steps."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_requests):
+        k = int(rng.integers(1, max_rods + 1))
+        rods = rng.integers(0, n_templates, k)
+        steps = int(np.rint(np.exp(rng.uniform(np.log(min_steps), np.log(max_steps)))))
+        out.append((rods, steps))
+    return out
 
 
 def apply_hooks(target, wl):

@@ -169,6 +169,60 @@ int erod_reorthonormalize(erod_batch* b);
  * launches are included. */
 int erod_last_kernel_ms(const erod_batch* b, double* ms, uint64_t* launches);
 
+/* ---- continuous batching: slots, budgets, rod copies ------------------------ */
+
+/* Active length of rod slot `rod`: 0 (an empty slot) or 2 ... its capacity (the
+ * n it was created with).  Per-rod transfers and the fused path use the active
+ * length; the op-by-op path needs every rod at full capacity
+ * (EROD_ERR_UNSUPPORTED otherwise).  No counterpart in the reference, whose
+ * blocks are rebuilt to change their rods (Block.hpp:568-611); for callers that
+ * reuse device slots for new rods. */
+int erod_set_rod_active_length(erod_batch* b, uint64_t rod, uint64_t n);
+int erod_get_rod_active_length(const erod_batch* b, uint64_t rod, uint64_t* n);
+/* Step budget of a rod on the fused path: a fused call of n steps advances the
+ * rod by min(n, budget) steps and spends them; a rod with no budget left keeps
+ * its state.  UINT64_MAX (the default) is unlimited.  No counterpart in the
+ * reference (one clock for all systems); for callers that mix rods with
+ * different horizons in one batch. */
+int erod_set_rod_step_budget(erod_batch* b, uint64_t rod, uint64_t steps);
+int erod_get_rod_step_budget(const erod_batch* b, uint64_t rod, uint64_t* steps);
+/* Copy whole rods on the device: every field, the rod clock, hooks, damping
+ * rates and dt of src_rods[i] go into slot dst_rods[i] of another batch on the
+ * same device; the slot's active length becomes the source rod's and its budget
+ * unlimited.  No counterpart in the reference; for admission of requests into
+ * a pool of slots. */
+int erod_copy_rods(erod_batch* dst, const uint64_t* dst_rods, erod_batch* src, const uint64_t* src_rods, uint64_t count);
+
+/* ---- serving: continuous batching of requests ------------------------------- */
+
+/* A server owns a pool batch of empty rod slots in size classes (n_classes
+ * capacities and slot counts) and serves requests made of rods of `library`
+ * (which must outlive it).  The pool's HBM (all fields of all slots) must fit
+ * `hbm_budget_bytes`.  Every tick admits queued requests first come first
+ * served (all rods of a request at once, each into the smallest free slot that
+ * holds it), advances every slot with budget left by up to `steps_per_tick`
+ * fused steps at `precision`, waits for the device, and reports finished
+ * requests.  No counterpart in the reference; its nearest analogue is the
+ * first-fit blocking of rods at emplace time (BlockingPolicy.hpp:202-222). */
+typedef struct erod_server erod_server;
+int erod_server_create(erod_batch* library, const uint64_t* class_capacity, const uint64_t* class_slots, int n_classes,
+                       int precision, int steps_per_tick, double hbm_budget_bytes, erod_server** out);
+int erod_server_destroy(erod_server* s);
+/* The pool batch (for its tile configuration or environment). */
+int erod_server_pool(erod_server* s, erod_batch** pool);
+/* Queue request `id`: the given library rods, advanced `steps` steps each with
+ * their own dt.  Rejected if it could not fit the empty pool. */
+int erod_server_submit(erod_server* s, uint64_t id, const uint64_t* library_rods, uint64_t n_rods, uint64_t steps);
+/* One tick; up to `capacity` finished request ids go to done_ids (the rest are
+ * reported by later ticks). */
+int erod_server_tick(erod_server* s, uint64_t* done_ids, uint64_t capacity, uint64_t* n_done);
+/* Field of rod `rod` (index within the request) of a finished request. */
+int erod_server_read(erod_server* s, uint64_t id, uint64_t rod, int field, double* host);
+/* Free a finished request's slots. */
+int erod_server_release(erod_server* s, uint64_t id);
+int erod_server_status(const erod_server* s, uint64_t* queued, uint64_t* running, uint64_t* active_elements,
+                       uint64_t* pool_bytes);
+
 /* ---- synchronisation and errors ------------------------------------------ */
 int erod_synchronize(erod_batch* b);
 /* Flags a non-finite value in x, v, Q or w of any rod (0/1 per rod). */

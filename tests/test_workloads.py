@@ -53,7 +53,7 @@ def test_contact_c4_geometry_and_dt():
     assert np.isclose(st["ReferenceElementLength"].sum(), 1.0)
     # 0.01 (L / n) sqrt(1e6 / 5e3) / sqrt(E / rho) with L = 1, n = 128, E = 1e6, rho = 1e3
     assert np.isclose(wl.dt, 0.01 / 128 * np.sqrt(200.0) / np.sqrt(1e3))
-    assert wl.contact == (1e3, 10.0, 0.3, 1e2, 0.0, 0.01) and wl.gravity == (0.0, 0.0, -9.81)
+    assert wl.contact == (1e3, 10.0, 0.3, 10.0, 0.0, 0.01) and wl.gravity == (0.0, 0.0, -9.81)
     # same seed, same rods; rods differ in their rest curvature
     again = contact_c4(n_rods=5, n=128)
     assert np.array_equal(again.states[2]["ReferenceCurvature"], wl.states[2]["ReferenceCurvature"])
