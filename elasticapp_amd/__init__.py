@@ -98,6 +98,8 @@ def load_library():
         "erod_last_kernel_ms": (i32, [vp, P(dbl), P(u64)]),
         "erod_synchronize": (i32, [vp]),
         "erod_check_finite": (i32, [vp, P(ctypes.c_uint8)]),
+        "erod_save": (i32, [vp, ctypes.c_char_p]),
+        "erod_load": (i32, [ctypes.c_char_p, P(vp)]),
         "erod_set_rod_active_length": (i32, [vp, u64, u64]),
         "erod_get_rod_active_length": (i32, [vp, u64, P(u64)]),
         "erod_set_rod_step_budget": (i32, [vp, u64, u64]),
@@ -322,6 +324,34 @@ class DeviceBlock:
 
     def synchronize(self):
         _check(_LIB.erod_synchronize(self._h))
+
+    # -- checkpoints --
+    def save(self, path):
+        """Write the whole batch (fields, clocks, rod records, budgets, environment)."""
+        _check(_LIB.erod_save(self._h, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, path):
+        """A new batch continuing from a checkpoint written by `save`."""
+        from .checkpoint import read_header
+        L = load_library()
+        damped, n_elems = read_header(path)
+        h = ctypes.c_void_p()
+        _check(L.erod_load(os.fsencode(path), ctypes.byref(h)))
+        self = cls.__new__(cls)
+        self._h = h
+        self.n_elems = n_elems
+        self.damped = damped
+        e = ctypes.c_uint64()
+        _check(L.erod_batch_slots(self._h, ctypes.byref(e)))
+        self.element_slots = e.value
+        self.starts = []
+        for r in range(len(n_elems)):
+            s_ = ctypes.c_uint64()
+            _check(L.erod_rod_start(self._h, r, ctypes.byref(s_)))
+            self.starts.append(s_.value)
+        self.time = float(self.rod_time()[0])  # the op-by-op clock
+        return self
 
     # -- slots, budgets, rod copies --
     def set_active_length(self, rod, n):

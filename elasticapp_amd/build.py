@@ -28,6 +28,7 @@ SOURCES = {
     "erod_fused.hip": [],
     "erod_capi.hip": [],
     "erod_serve.hip": [],
+    "erod_io.hip": [],
 }
 
 

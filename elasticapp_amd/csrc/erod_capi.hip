@@ -390,7 +390,8 @@ int erod_step(erod_batch* b, double* time, double dt, uint64_t n) {
     if ((rc = ops_kinematic_update(b, dt))) return rc;
   }
   *time = t;
-  return EROD_OK;
+  // every rod clock follows the caller's clock, so checkpoints carry it
+  return ops_set_rod_clocks(b, t);
 }
 
 int erod_set_rod_dt(erod_batch* b, const double* dt_per_rod) {

@@ -179,6 +179,7 @@ namespace erod {
   int ops_dynamic_update(erod_batch* b, double dt);
   int ops_fill_ghosts(erod_batch* b, int field);
   int ops_mark_ghost_nodes(erod_batch* b);
+  int ops_set_rod_clocks(erod_batch* b, double t);
   int ops_apply_post_kinematic(erod_batch* b);
   int ops_apply_pre_dynamic(erod_batch* b, double time);
   int ops_apply_post_dynamic(erod_batch* b);

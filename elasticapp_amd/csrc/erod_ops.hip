@@ -378,6 +378,11 @@ namespace erod {
 #define L(kernel, n, ...) hipLaunchKernelGGL(kernel, dim3(grid_for(n)), dim3(kThreads), 0, b->stream, __VA_ARGS__)
 #define F(id) (b->d_field[id])
 
+  int ops_set_rod_clocks(erod_batch* b, double t) {
+    L(k_fill, b->n_rods, b->d_time, t, b->n_rods);
+    return hip_check(hipGetLastError(), "set rod clocks");
+  }
+
   int ops_mark_ghost_nodes(erod_batch* b) {
     EROD_HIP(hipMemsetAsync(b->d_node_is_ghost, 0, b->E + 1, b->stream));
     if (!b->ghost_nodes.empty()) L(k_mark_ghost_nodes, b->ghost_nodes.size(), b->d_node_is_ghost, b->d_ghost_nodes,

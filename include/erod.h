@@ -131,7 +131,8 @@ int erod_apply_pre_dynamic(erod_batch* b, double time);
 int erod_apply_post_dynamic(erod_batch* b, double time);
 
 /* n full position-Verlet steps through the op-by-op kernels
- * (SymplecticSteppers.hpp:133-217); *time advances by n dt. */
+ * (SymplecticSteppers.hpp:133-217); *time advances by n dt and every rod
+ * clock (erod_get_rod_time) is set to it. */
 int erod_step(erod_batch* b, double* time, double dt, uint64_t n);
 
 /* ---- fused / persistent path ---------------------------------------------- */
@@ -222,6 +223,17 @@ int erod_server_read(erod_server* s, uint64_t id, uint64_t rod, int field, doubl
 int erod_server_release(erod_server* s, uint64_t id);
 int erod_server_status(const erod_server* s, uint64_t* queued, uint64_t* running, uint64_t* active_elements,
                        uint64_t* pool_bytes);
+
+/* ---- checkpoints ------------------------------------------------------------ */
+
+/* Write everything the batch holds (all fields, rod clocks, rod records with
+ * dt, damping and hooks, active lengths, step budgets, environment) to one
+ * binary file; erod_load creates a batch from it that continues bit for bit.
+ * Replaces the block archive of elastica/Systems/Block/Serialize.hpp:37-62 and
+ * elastica/Systems/CosseratRods/Serialize.hpp:41-72 (variables + ghost node
+ * buffer).  The layout is documented in elasticapp_amd/csrc/erod_io.hip. */
+int erod_save(erod_batch* b, const char* path);
+int erod_load(const char* path, erod_batch** out);
 
 /* ---- synchronisation and errors ------------------------------------------ */
 int erod_synchronize(erod_batch* b);

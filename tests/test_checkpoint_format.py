@@ -1,0 +1,55 @@
+"""The numpy checkpoint reader against a file packed by hand from the layout
+documented in erod_io.hip (no GPU)."""
+import struct
+
+import numpy as np
+import pytest
+
+import elasticapp_amd as ea
+from elasticapp_amd import checkpoint
+
+
+def _pack(n_elems):
+    ea.load_library()
+    n_rods = len(n_elems)
+    E = sum(n_elems) + 2 * n_rods - 2
+    b = bytearray(b"ERODCKP1")
+    b += struct.pack("<IIQQ", 1, 1, n_rods, E)
+    b += np.array(n_elems, "<u8").tobytes() + np.array(n_elems, "<u8").tobytes()
+    b += np.array([7] * n_rods, "<u4").tobytes() + b"\x01"
+    b += np.arange(n_rods, dtype="<f8").tobytes()
+    for r in range(n_rods):
+        b += struct.pack("<3dI3d9d3di2d", 1e-3 * (r + 1), 0.1, 0.2, 2, 0, 0, 0, *([0.0] * 9), 1.0, 2.0, 3.0, 1, 0.0, 5.0)
+    b += struct.pack("<i3di6d", 1, 0.0, 0.0, -9.81, 0, 0, 0, 0, 0, 0, 0)
+    b += struct.pack("<I", len(ea.FIELDS))
+    fields = {}
+    for name, (fid, nc, st) in sorted(ea.FIELDS.items(), key=lambda kv: kv[1][0]):
+        cnt = E + 1 if st == 0 else (E if st == 1 else E - 1)
+        a = np.random.default_rng(fid).normal(size=(nc, cnt))
+        fields[name] = a
+        b += struct.pack("<IIQ", fid, nc, cnt) + a.astype("<f8").tobytes()
+    b += b"ERODEND1"
+    return bytes(b), fields
+
+
+def test_reader_parses_the_documented_layout(tmp_path):
+    data, fields = _pack([5, 3, 9])
+    p = tmp_path / "x.ckpt"
+    p.write_bytes(data)
+    assert checkpoint.read_header(str(p)) == (True, [5, 3, 9])
+    ck = checkpoint.read(str(p))
+    assert list(ck["starts"]) == [0, 7, 12] and ck["budgets"] and list(ck["steps_left"]) == [7, 7, 7]
+    assert ck["rods"][2]["dt"] == 3e-3 and list(ck["rods"][1]["force"]) == [1.0, 2.0, 3.0] and ck["rods"][0]["ramp"] == 1
+    assert ck["env"]["gravity"] and not ck["env"]["contact"] and ck["env"]["g"][2] == -9.81
+    for name, a in fields.items():
+        assert np.array_equal(ck["fields"][name], a), name
+    x = checkpoint.rod_field(ck, "Position", 1)
+    assert np.array_equal(x, fields["Position"][:, 7:11])
+
+
+def test_reader_rejects_truncation(tmp_path):
+    data, _ = _pack([4, 4])
+    p = tmp_path / "t.ckpt"
+    p.write_bytes(data[:-20])
+    with pytest.raises(ValueError):
+        checkpoint.read(str(p))

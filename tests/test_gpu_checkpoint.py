@@ -1,0 +1,91 @@
+"""Checkpoints of device batches: a run resumed from erod_save / erod_load
+continues bit for bit (op-by-op and both fused precisions, with hooks, step
+budgets and shortened slots), and the numpy reader returns exactly what the
+device holds."""
+import numpy as np
+import pytest
+
+from elasticapp_amd import FIELDS, FP32_BULK, FP64, DeviceBlock, ErodError
+from elasticapp_amd import checkpoint
+from elasticapp_amd.workloads import build_block
+from tests.gpu_helpers import STATE, random_workload
+
+pytestmark = pytest.mark.gpu
+
+
+def _hooked():
+    wl = random_workload(7, 8, 70, seed=53, steps=0, forces=False)
+    for r in range(3):
+        st = wl.states[r]
+        wl.clamps.append((r, st["Position"][:, 0].copy(), st["Director"][:, 0].copy()))
+        wl.forces.append((r, np.array([0.2, -0.1, 0.3]), 1 + r % 2, 0.0, wl.dt * 120))
+    wl.gravity = (0.0, 0.0, -9.81)
+    zmin = min(st["Position"][2].min() for st in wl.states)
+    wl.contact = (1e3, 10.0, 0.3, 10.0, zmin + 0.01, 0.0)
+    return wl
+
+
+def _same(a, b, tags):
+    for r in range(len(a.n_elems)):
+        if a.active_length(r) == 0:
+            continue
+        for t in tags:
+            assert np.array_equal(a.get_rod(t, r), b.get_rod(t, r)), (r, t)
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32_BULK])
+def test_fused_run_resumes_bitwise(tmp_path, precision):
+    wl = _hooked()
+    a = build_block(wl)
+    a.set_step_budget(1, 150)
+    a.set_active_length(4, max(2, wl.n_elems[4] - 5))
+    a.step_fused(100, precision)
+    path = str(tmp_path / "run.ckpt")
+    a.save(path)
+    a.step_fused(100, precision)
+    b = DeviceBlock.load(path)
+    assert b.n_elems == a.n_elems and b.step_budget(1) == 50 and b.active_length(4) == a.active_length(4)
+    b.step_fused(100, precision)
+    _same(a, b, STATE)
+    assert np.array_equal(a.rod_time(), b.rod_time())
+    assert b.step_budget(1) == 0 and b.step_budget(0) is None
+
+
+def test_op_by_op_run_resumes_bitwise(tmp_path):
+    wl = _hooked()
+    a = build_block(wl)
+    a.step(wl.dt, 60)
+    path = str(tmp_path / "ops.ckpt")
+    a.save(path)
+    t_saved = a.time
+    a.step(wl.dt, 60)
+    b = DeviceBlock.load(path)
+    assert b.time == t_saved
+    b.step(wl.dt, 60)
+    assert b.time == a.time
+    _same(a, b, list(FIELDS))
+
+
+def test_reader_returns_device_contents(tmp_path):
+    wl = _hooked()
+    a = build_block(wl)
+    a.step_fused(40, FP32_BULK)
+    path = str(tmp_path / "r.ckpt")
+    a.save(path)
+    ck = checkpoint.read(path)
+    assert list(ck["n_elems"]) == a.n_elems and ck["damped"]
+    assert np.array_equal(ck["rod_time"], a.rod_time())
+    for tag in FIELDS:
+        assert np.array_equal(ck["fields"][tag], a.get_field(tag)), tag
+    for r in (0, 3, 6):
+        for tag in STATE:
+            assert np.array_equal(checkpoint.rod_field(ck, tag, r), a.get_rod(tag, r)), (r, tag)
+    assert ck["env"]["gravity"] and ck["env"]["contact"] and ck["env"]["plane"]["k"] == 1e3
+    assert ck["rods"][0]["flags"] & 1 and ck["rods"][1]["ramp"] == 2 and ck["rods"][0]["dt"] == wl.dt
+
+
+def test_load_rejects_a_foreign_file(tmp_path):
+    p = tmp_path / "bad.ckpt"
+    p.write_bytes(b"ERODCKP1" + b"\x00" * 40)
+    with pytest.raises((ErodError, ValueError)):
+        DeviceBlock.load(str(p))
