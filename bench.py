@@ -150,7 +150,8 @@ def c4_leg(args, torch):
     lanes, epl, spl = blk.fused_config()
     flags = blk.check_finite()
     assert not flags.any(), "non-finite state in %d contact rods" % int(flags.sum())
-    zmin = float(blk.get_field("Position")[2].min())
+    z = blk.get_field("Position")[2]
+    zmin = float(min(z[s0:s0 + k + 1].min() for s0, k in zip(blk.starts, n)))  # rod nodes, not ghosts
     v = elems * args.c4_steps / el
     bytes_launch = float(b32(n).sum()) * (args.c4_steps // launches)
     achieved = bytes_launch / (kms * 1e-3) / 1e9
