@@ -255,6 +255,50 @@ namespace erod {
       }
     }
 
+    // the constants of a thread's P elements, one field line at a time: the P
+    // loads of a line fall in the same cache lines and are issued back to back
+    template <typename R, int P>
+    __device__ __forceinline__ void load_consts(const Consts<R>& C, const uint64_t (&s)[P], uint64_t E, const bool (&ok)[P],
+                                                const bool (&has_vor)[P], ElemConst<R> (&K)[P]) {
+      const uint64_t V = E - 1;
+#pragma unroll
+      for (int j = 0; j < P; ++j)
+        if (ok[j]) K[j].m = C.m[s[j]];
+#pragma unroll
+      for (int j = 0; j < P; ++j)
+        if (ok[j]) K[j].mn = C.m[s[j] + 1];
+#pragma unroll
+      for (int j = 0; j < P; ++j)
+        if (ok[j]) K[j].invm = C.invm[s[j]];
+#pragma unroll
+      for (int j = 0; j < P; ++j)
+        if (ok[j]) K[j].l0 = C.l0[s[j]];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int j = 0; j < P; ++j)
+          if (ok[j]) K[j].sig0[c] = C.sig0[c * E + s[j]];
+#pragma unroll
+        for (int j = 0; j < P; ++j)
+          if (ok[j]) K[j].S[c] = C.S[c * E + s[j]];
+#pragma unroll
+        for (int j = 0; j < P; ++j)
+          if (ok[j]) K[j].J[c] = C.J[c * E + s[j]];
+#pragma unroll
+        for (int j = 0; j < P; ++j)
+          if (ok[j]) K[j].iJ[c] = C.iJ[c * E + s[j]];
+      }
+#pragma unroll
+      for (int j = 0; j < P; ++j) K[j].D0 = ok[j] && has_vor[j] ? C.D0[s[j]] : R(1);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int j = 0; j < P; ++j) K[j].kap0[c] = ok[j] && has_vor[j] ? C.kap0[c * V + s[j]] : R(0);
+#pragma unroll
+        for (int j = 0; j < P; ++j) K[j].B[c] = ok[j] && has_vor[j] ? C.B[c * V + s[j]] : R(0);
+      }
+    }
+
     // per-element values carried from the first to the second half of a step
     template <typename R>
     struct Mid {
@@ -389,6 +433,9 @@ namespace erod {
                                                   const uint32_t* __restrict__ tile_lane0, double* __restrict__ rod_time,
                                                   uint32_t* __restrict__ steps_left, Env env, int nsteps, uint64_t E) {
       constexpr bool kCache = sizeof(R) == 4;  // FP32 constants stay in registers for the launch
+      // FP32-bulk: the closing half step of one step and the opening half step
+      // of the next (same v and w) are taken as one update of dt within a launch
+      constexpr bool kMerge = sizeof(R) == 4;
       const uint64_t N = E + 1;
       const Tile tile = tiles[blockIdx.x];
       const int tid = threadIdx.x;
@@ -412,6 +459,7 @@ namespace erod {
       R tail_m = R(0), tail_im = R(0);
       double xt[3] = {0, 0, 0}, vt[3] = {0, 0, 0};
       int tailj = -1;
+      // pass 1: which rod and slot each of the thread's elements is
 #pragma unroll
       for (int j = 0; j < P; ++j) {
         const uint32_t el = uint32_t(tid) * P + j;
@@ -435,32 +483,50 @@ namespace erod {
         rflags[j] = ri.flags;
         gfj[j] = static_cast<R>(ri.gamma_f);
         gtj[j] = static_cast<R>(ri.gamma_t);
-        if (valid[j]) {
+      }
+      // pass 2: the state, one field line at a time over the P elements (their
+      // slots are contiguous within a rod, so the P loads of a line share cache
+      // lines and are issued back to back)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int j = 0; j < P; ++j) x[j][c] = valid[j] ? X[c * N + slot[j]] : 0.0;
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int j = 0; j < P; ++j) v[j][c] = valid[j] ? Vel[c * N + slot[j]] : 0.0;
+      }
+#pragma unroll
+      for (int m = 0; m < 9; ++m) {
+#pragma unroll
+        for (int j = 0; j < P; ++j) q[j][m] = valid[j] ? Qd[m * E + slot[j]] : ((m % 4 == 0) ? 1.0 : 0.0);
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int j = 0; j < P; ++j) w[j][c] = valid[j] ? W[c * E + slot[j]] : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        if (valid[j] && kk[j] == nn[j] - 1) {
+          tailj = j;
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            x[j][c] = X[c * N + slot[j]];
-            v[j][c] = Vel[c * N + slot[j]];
-            w[j][c] = W[c * E + slot[j]];
+            xt[c] = X[c * N + slot[j] + 1];
+            vt[c] = Vel[c * N + slot[j] + 1];
           }
-#pragma unroll
-          for (int m = 0; m < 9; ++m) q[j][m] = Qd[m * E + slot[j]];
-          if (kk[j] == nn[j] - 1) {
-            tailj = j;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-              xt[c] = X[c * N + slot[j] + 1];
-              vt[c] = Vel[c * N + slot[j] + 1];
-            }
-            tail_m = C.m[slot[j] + 1];
-            tail_im = C.invm[slot[j] + 1];
-          }
-          if constexpr (kCache) load_const<R>(C, slot[j], E, kk[j] < nn[j] - 1, kc[kCache ? j : 0]);
-        } else {
-#pragma unroll
-          for (int c = 0; c < 3; ++c) x[j][c] = v[j][c] = w[j][c] = 0.0;
-#pragma unroll
-          for (int m = 0; m < 9; ++m) q[j][m] = (m % 4 == 0) ? 1.0 : 0.0;
+          tail_m = C.m[slot[j] + 1];
+          tail_im = C.invm[slot[j] + 1];
         }
+      }
+      if constexpr (kCache) {
+        bool has_vor[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) has_vor[j] = kk[j] < nn[j] - 1;
+#pragma unroll
+        for (int j = 0; j < P; ++j)
+          if (valid[j]) load_const<R>(C, slot[j], E, has_vor[j], kc[kCache ? j : 0]);
       }
 
       for (int step = 0; step < nsteps; ++step) {
@@ -470,16 +536,20 @@ namespace erod {
 #pragma unroll
         for (int j = 0; j < P; ++j) live[j] = valid[j] && uint32_t(step) < lim[j];
         // ---- kinematic half step, time, post-kinematic clamp --------------------
+        // (FP32-bulk: after the first step of a launch this half step was taken
+        // together with the closing half step of the previous one)
 #pragma unroll
         for (int j = 0; j < P; ++j) {
           if (!live[j]) continue;
-          const double ch = 0.5 * dt[j];
+          if (!kMerge || step == 0) {
+            const double ch = 0.5 * dt[j];
 #pragma unroll
-          for (int c = 0; c < 3; ++c) x[j][c] = maddd<R>(ch, v[j][c], x[j][c]);
-          rodrigues<R>(q[j], w[j], ch);
-          if (j == tailj)
+            for (int c = 0; c < 3; ++c) x[j][c] = maddd<R>(ch, v[j][c], x[j][c]);
+            rodrigues<R>(q[j], w[j], ch);
+            if (j == tailj)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) xt[c] = maddd<R>(ch, vt[c], xt[c]);
+              for (int c = 0; c < 3; ++c) xt[c] = maddd<R>(ch, vt[c], xt[c]);
+          }
           tm[j] += 0.5 * dt[j];
           if (kk[j] == 0 && (rflags[j] & 1u)) {
             const RodInfo& ri = rods[rodi[j]];
@@ -686,11 +756,14 @@ namespace erod {
           }
         }
         // ---- time, kinematic half step ------------------------------------------
+        // FP32-bulk: unless this is the rod's last step of the launch, also take
+        // the next step's opening half step (same v and w: one update of dt)
 #pragma unroll
         for (int j = 0; j < P; ++j) {
           if (!live[j]) continue;
           tm[j] += 0.5 * dt[j];
-          const double ch = 0.5 * dt[j];
+          const bool both = kMerge && uint32_t(step + 1) < lim[j] && step + 1 < nsteps;
+          const double ch = both ? dt[j] : 0.5 * dt[j];
 #pragma unroll
           for (int c = 0; c < 3; ++c) x[j][c] = maddd<R>(ch, v[j][c], x[j][c]);
           rodrigues<R>(q[j], w[j], ch);
@@ -700,19 +773,35 @@ namespace erod {
         }
       }
 
-      // ---- write back -------------------------------------------------------------
+      // ---- write back (one field line at a time over the P elements) --------------
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int j = 0; j < P; ++j)
+          if (valid[j]) X[c * N + slot[j]] = x[j][c];
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int j = 0; j < P; ++j)
+          if (valid[j]) Vel[c * N + slot[j]] = v[j][c];
+      }
+#pragma unroll
+      for (int m = 0; m < 9; ++m) {
+#pragma unroll
+        for (int j = 0; j < P; ++j)
+          if (valid[j]) Qd[m * E + slot[j]] = q[j][m];
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int j = 0; j < P; ++j)
+          if (valid[j]) W[c * E + slot[j]] = w[j][c];
+      }
 #pragma unroll
       for (int j = 0; j < P; ++j) {
         if (!valid[j]) continue;
         const uint64_t s = slot[j];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          X[c * N + s] = x[j][c];
-          Vel[c * N + s] = v[j][c];
-          W[c * E + s] = w[j][c];
-        }
-#pragma unroll
-        for (int m = 0; m < 9; ++m) Qd[m * E + s] = q[j][m];
         if (j == tailj) {
 #pragma unroll
           for (int c = 0; c < 3; ++c) {

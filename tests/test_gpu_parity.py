@@ -11,7 +11,8 @@ Tolerances
     room for the device math library).
   * FP32-bulk path: position error <= 1e-6 relative to the position scale.
   * both fused paths: bitwise independent of the tile shape (lanes per tile,
-    elements per thread, steps per launch) and of the other rods in the batch.
+    elements per thread) and of the other rods in the batch; the FP64 path
+    also of the steps per launch.
 """
 import numpy as np
 import pytest
@@ -121,10 +122,12 @@ SHAPES = [(256, 1), (512, 1), (512, 2), (1024, 2), (512, 4), (1024, 4)]
 @pytest.mark.parametrize("lanes,epl", SHAPES)
 def test_fused_is_independent_of_tiling(lanes, epl, precision):
     """Per-rod arithmetic does not depend on the tile, the thread or the
-    position within a thread a rod lands in."""
+    position within a thread a rod lands in.  (FP32-bulk takes the two half
+    steps between consecutive steps of a launch as one, so runs compared
+    bitwise use the same steps per launch.)"""
     wl = random_workload(30, 4, 250, seed=11, steps=50)
     ref = device_block(wl)
-    ref.set_fused_config(256, 1, 25)
+    ref.set_fused_config(256, 1, 10)
     ref.step_fused(wl.steps, precision)
     blk = device_block(wl)
     blk.set_fused_config(lanes, epl, 10)
@@ -148,6 +151,34 @@ def test_autotune_keeps_state_and_picks_a_shape():
     for r in range(len(wl.n_elems)):
         for t in STATE:
             assert np.array_equal(a.get_rod(t, r), b.get_rod(t, r)), (r, t)
+
+
+def test_fp64_fused_is_independent_of_steps_per_launch():
+    wl = random_workload(10, 4, 120, seed=19, steps=60)
+    a = device_block(wl)
+    a.set_fused_config(0, 0, 60)
+    a.step_fused(60, FP64)
+    b = device_block(wl)
+    b.set_fused_config(0, 0, 7)
+    b.step_fused(60, FP64)
+    for r in range(len(wl.n_elems)):
+        for t in STATE:
+            assert np.array_equal(a.get_rod(t, r), b.get_rod(t, r)), (r, t)
+
+
+def test_fp32_bulk_launch_segmentation_is_a_rounding_effect():
+    """Merging the half steps inside a launch changes FP32-bulk results only
+    at FP64 rounding level."""
+    wl = random_workload(10, 4, 120, seed=19, steps=60)
+    a = device_block(wl)
+    a.set_fused_config(0, 0, 60)
+    a.step_fused(60, FP32_BULK)
+    b = device_block(wl)
+    b.set_fused_config(0, 0, 1)
+    b.step_fused(60, FP32_BULK)
+    for r in range(len(wl.n_elems)):
+        x, y = a.get_rod("Position", r), b.get_rod("Position", r)
+        assert np.abs(x - y).max() <= 1e-12 * max(1.0, np.abs(y).max()), r
 
 
 def test_fused_graph_replay_equals_direct():

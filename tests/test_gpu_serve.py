@@ -23,12 +23,15 @@ def _library(n_rods=24, seed=41):
     return wl, blk
 
 
-def _direct(wl, rods, steps, precision, dts):
-    """The given library rods stepped `steps` fused steps in a batch of their own."""
+def _direct(wl, rods, steps, precision, dts, steps_per_launch=100):
+    """The given library rods stepped `steps` fused steps in a batch of their
+    own, in launches of `steps_per_launch` steps (FP32-bulk results depend on
+    the launch segmentation at rounding level)."""
     sub = Workload("sub", [wl.n_elems[r] for r in rods], [wl.states[r] for r in rods], wl.dt, steps,
                    forces=[(i,) + tuple(wl.forces[r][1:]) for i, r in enumerate(rods)])
     blk = build_block(sub)
     blk.set_rod_dt(np.array([dts[r] for r in rods]))
+    blk.set_fused_config(0, 0, steps_per_launch)
     blk.step_fused(steps, precision)
     return blk
 
@@ -46,6 +49,7 @@ def test_step_budgets(precision):
     for r, b in enumerate(budgets):
         ref = build_block(Workload("one", [wl.n_elems[r]], [wl.states[r]], wl.dt, 0,
                                    forces=[(0,) + tuple(wl.forces[r][1:])]))
+        ref.set_fused_config(0, 0, 30)
         n = 120 if b is None else min(b, 120)
         if n:
             ref.step_fused(n, precision)
@@ -120,7 +124,7 @@ def test_server_mix_equals_alone(precision):
         got = []
         while not got:
             got = alone.tick()
-        ref = _direct(wl, rods, steps, precision, dts)
+        ref = _direct(wl, rods, steps, precision, dts, steps_per_launch=50)
         for k in range(len(rods)):
             for t in STATE:
                 a = alone.read(rid, k, t)
