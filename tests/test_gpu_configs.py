@@ -1,0 +1,65 @@
+"""The small configurations run in full against the CPU oracle.
+
+  * Timoshenko cantilever (1 rod x 100 elements, clamped, tip force, damping
+    0.1, dt 3e-4): 50,000 fused FP64 steps and 2,000 op-by-op steps against
+    the oracle, and the FP32-bulk tip within 1e-6 of the rod length;
+  * 1024 rods x 64 elements with random rest curvature and velocities:
+    after 10,000 fused FP64 steps positions and directors within 1e-10 of
+    the oracle, velocities and angular velocities within 1e-6 of their
+    (decayed) size; 500 op-by-op steps within 1e-12.  The velocity bound is
+    the batch's own conditioning: in the oracle alone, a last-bit change of
+    one velocity component per rod grows to ~1e-9 of the angular velocities over these steps
+    (test_oracle_kats.py::test_free_rods_amplify_a_last_bit_change), and the
+    device acos / sin / cos differ from the host's in the last bit every step.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from elasticapp_amd import FP32_BULK, FP64
+from elasticapp_amd.workloads import build_block, timoshenko_c0, uniform_c1
+from tests.gpu_helpers import STATE, oracle_system, rel_state_error
+
+pytestmark = pytest.mark.gpu
+
+THREADS = max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def test_timoshenko_against_oracle():
+    wl = timoshenko_c0(steps=50000)
+    fused = build_block(wl)
+    fused.step_fused(wl.steps, FP64)
+    sysm = oracle_system(wl, max_block_elems=1024)
+    sysm.step(wl.dt, wl.steps, THREADS)
+    err = rel_state_error(fused, sysm)
+    assert max(err.values()) <= 1e-10, err
+    f32 = build_block(wl)
+    f32.step_fused(wl.steps, FP32_BULK)
+    tip32, tip = f32.get_rod("Position", 0)[:, -1], sysm.rod_field(0, "Position")[:, -1]
+    assert np.abs(tip32 - tip).max() <= 1e-6 * wl.info["length"], (tip32, tip)
+    # the op-by-op path over the first 2,000 steps
+    ops = build_block(wl)
+    ops.step(wl.dt, 2000)
+    s2 = oracle_system(wl, max_block_elems=1024)
+    s2.step(wl.dt, 2000, THREADS)
+    err = rel_state_error(ops, s2)
+    assert max(err.values()) <= 1e-12, err
+
+
+def test_uniform_batch_against_oracle():
+    wl = uniform_c1(steps=10000)
+    fused = build_block(wl)
+    fused.step_fused(wl.steps, FP64)
+    assert not fused.check_finite().any()
+    sysm = oracle_system(wl, max_block_elems=1024)
+    sysm.step(wl.dt, wl.steps, THREADS)
+    err = rel_state_error(fused, sysm)
+    assert err["Position"] <= 1e-10 and err["Director"] <= 1e-10, err
+    assert err["Velocity"] <= 1e-6 and err["AngularVelocity"] <= 1e-6, err
+    ops = build_block(wl)
+    ops.step(wl.dt, 500)
+    s2 = oracle_system(wl, max_block_elems=1024)
+    s2.step(wl.dt, 500, THREADS)
+    err = rel_state_error(ops, s2)
+    assert max(err.values()) <= 1e-12, err

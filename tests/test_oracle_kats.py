@@ -259,3 +259,28 @@ def test_sliding_friction_decelerates_at_mu_g():
     vcm = (v * mass).sum() / mass.sum()
     a = 0.3 * 9.81 / 1.0
     assert abs(vcm - ((1.0 + a) * np.exp(-1.0 * T) - a)) <= 1e-3
+
+
+def test_free_rods_amplify_a_last_bit_change():
+    """Conditioning of the 1024 x 64 batch (32 of its rods): one ulp added to
+    one velocity component of each rod grows to ~1e-9 of the angular
+    velocities over 10,000 steps while positions stay within ~1e-14.  Bounds a GPU run whose
+    transcendentals differ from the host's in the last bit can be held to."""
+    from elasticapp_amd.workloads import Workload, uniform_c1
+    from tests.gpu_helpers import oracle_system
+    wl = uniform_c1(steps=10000)
+    a = oracle_system(Workload("a", wl.n_elems[:32], wl.states[:32], wl.dt, 0), max_block_elems=1024)
+    st2 = [dict(s) for s in wl.states[:32]]
+    for st in st2:
+        v = st["Velocity"].copy()
+        v[0, 5] = np.nextafter(v[0, 5], np.inf)
+        st["Velocity"] = v
+    b = oracle_system(Workload("b", wl.n_elems[:32], st2, wl.dt, 0), max_block_elems=1024)
+    a.step(wl.dt, 10000, 4)
+    b.step(wl.dt, 10000, 4)
+
+    def rel(t):
+        return max(np.abs(a.rod_field(r, t) - b.rod_field(r, t)).max() / np.abs(b.rod_field(r, t)).max()
+                   for r in range(32))
+    assert 1e-12 < rel("AngularVelocity") < 1e-7
+    assert rel("Position") < 1e-12
