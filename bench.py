@@ -246,8 +246,8 @@ def main():
     ap.add_argument("--rods", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (rods shard by rod with no "
-              "exchange; see the design notes)", file=sys.stderr)
+        print("bench.py runs on one GPU only: liberod has no multi-GPU path (rods are independent, so a batch "
+              "would shard by rods with no exchange per step)", file=sys.stderr)
         sys.exit(2)
     if args.steps % args.chunk or args.warmup % args.chunk or args.c4_steps % args.chunk:
         print("--steps, --warmup and --c4-steps must be multiples of --chunk", file=sys.stderr)
