@@ -114,8 +114,8 @@ int erod_batch_create(const uint64_t* n_elems, uint64_t n_rods, int damped, erod
   }
   *out = nullptr;
   for (uint64_t r = 0; r < n_rods; ++r)
-    if (n_elems[r] < 2) {
-      set_error("Cannot have rods with < 2 elements!");
+    if (n_elems[r] < 2 || n_elems[r] > (uint64_t(1) << 31)) {
+      set_error("Cannot have rods with < 2 elements! (or more than 2^31)");
       return EROD_ERR_INVALID;
     }
   int ndev = 0;
@@ -512,6 +512,14 @@ int erod_copy_rods(erod_batch* dst, const uint64_t* dst_rods, erod_batch* src, c
   if (dst->device != src->device || dst->damped != src->damped) {
     set_error("erod_copy_rods: batches on different devices or of different damping models");
     return EROD_ERR_INVALID;
+  }
+  {
+    std::vector<uint64_t> d(dst_rods, dst_rods + count);
+    std::sort(d.begin(), d.end());
+    if (std::adjacent_find(d.begin(), d.end()) != d.end()) {
+      set_error("erod_copy_rods: a destination slot appears twice");
+      return EROD_ERR_INVALID;
+    }
   }
   std::vector<CopyPair> pairs(count);
   for (uint64_t i = 0; i < count; ++i) {

@@ -62,8 +62,6 @@ namespace erod {
     __device__ __forceinline__ double rcp_(double x) { return 1.0 / x; }
     __device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
     __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
-    __device__ __forceinline__ float div_(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-    __device__ __forceinline__ double div_(double a, double b) { return a / b; }
 
     // a * b + c: one fused multiply-add in the FP32-bulk path, a separately
     // rounded product and sum in the FP64 path (which keeps the reference's
@@ -197,7 +195,7 @@ namespace erod {
     }
 
     template <>
-    __device__ __forceinline__ void curvature<float>(const double qc[9], const double qn[9], float D0, float kap[3]) {
+    __device__ __forceinline__ void curvature<float>(const double qc[9], const double qn[9], float iD0, float kap[3]) {
       const double tr = dot3_fma(qc, qn) + dot3_fma(qc + 3, qn + 3) + dot3_fma(qc + 6, qn + 6);
       // rows: d1 = q[0..2], d2 = q[3..5], d3 = q[6..8]
       const double s0 = skew3_fma(qc + 6, qn + 3, qc + 3, qn + 6);
@@ -205,14 +203,18 @@ namespace erod {
       const double s2 = skew3_fma(qc + 3, qn + 0, qc + 0, qn + 3);
       float a = static_cast<float>((3.0 - tr) * 0.25) + 5e-11f;
       a = fminf(fmaxf(a, 0.0f), 0.999999f);
-      float f;
+      // st = (theta / 2) / sin(theta) / D0 with a = sin^2(theta / 2):
+      // theta / (2 sin theta) = g(a) / 2, g(a) = asin(sqrt a) / (sqrt(a) sqrt(1 - a))
+      //                      = sum_n 4^n (n!)^2 / (2n + 1)! a^n = 1 + 2a/3 + 8a^2/15 + 16a^3/35 + 128a^4/315 + ...
+      // (the next term, 0.37 a^5, is below 1.2e-9 of g for a < 0.02)
+      float g;
       if (a < 0.02f) {
-        f = madd(a, madd(a, madd(a, madd(a, 35.0f / 1152.0f, 5.0f / 112.0f), 3.0f / 40.0f), 1.0f / 6.0f), 1.0f);
+        g = madd(a, madd(a, madd(a, madd(a, 128.0f / 315.0f, 16.0f / 35.0f), 8.0f / 15.0f), 2.0f / 3.0f), 1.0f);
       } else {
         const float sh = sqrt_(a);
-        f = asinf(sh) / sh;
+        g = asinf(sh) / (sh * sqrt_(1.0f - a));
       }
-      const float st = f * rcp_(2.0f * sqrt_(1.0f - a) * D0);
+      const float st = 0.5f * g * iD0;
       kap[0] = st * static_cast<float>(s0);
       kap[1] = st * static_cast<float>(s1);
       kap[2] = st * static_cast<float>(s2);
@@ -225,6 +227,7 @@ namespace erod {
       R m, invm, mn;  // node k mass and inverse, node k+1 mass
       R l0, sig0[3], S[3], J[3], iJ[3];
       R D0, kap0[3], B[3];  // Voronoi k (unused at the rod end)
+      R il0, iD0;           // 1 / l0, 1 / D0 (FP32-bulk only)
     };
 
     template <typename R>
@@ -252,6 +255,10 @@ namespace erod {
         K.D0 = R(1);
 #pragma unroll
         for (int c = 0; c < 3; ++c) K.kap0[c] = K.B[c] = R(0);
+      }
+      if constexpr (sizeof(R) == 4) {
+        K.il0 = 1.0f / K.l0;
+        K.iD0 = 1.0f / K.D0;
       }
     }
 
@@ -333,7 +340,14 @@ namespace erod {
         t[1] = static_cast<float>(d1) * il;
         t[2] = static_cast<float>(d2) * il;
       }
-      const R e = div_(l, K.l0);
+      R e, ie;
+      if constexpr (sizeof(R) == 8) {
+        e = l / K.l0;
+        ie = R(0);
+      } else {
+        e = l * K.il0;
+        ie = rcp_(e);
+      }
       // shear / stretch strain sigma = e (Q t) - z
       R qt[3];
 #pragma unroll
@@ -352,7 +366,6 @@ namespace erod {
 #pragma unroll
         for (int j = 0; j < 3; ++j) out.c[j] = qn[j] / e;
       } else {
-        const float ie = rcp_(e);
 #pragma unroll
         for (int j = 0; j < 3; ++j) out.c[j] = qn[j] * ie;
       }
@@ -363,11 +376,17 @@ namespace erod {
       R w[3] = {static_cast<R>(wk[0]), static_cast<R>(wk[1]), static_cast<R>(wk[2])};
       R jw[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) jw[c] = div_(K.J[c] * w[c], e);
+      for (int c = 0; c < 3; ++c) {
+        if constexpr (sizeof(R) == 8) jw[c] = (K.J[c] * w[c]) / e;
+        else jw[c] = (K.J[c] * w[c]) * ie;
+      }
       R lag[3];
       cross3(jw, w, lag);
       const R dv0 = static_cast<R>(vk1[0] - vk[0]), dv1 = static_cast<R>(vk1[1] - vk[1]), dv2 = static_cast<R>(vk1[2] - vk[2]);
-      const R rate = div_(madd(t[2], dv2, madd(t[1], dv1, t[0] * dv0)), e * K.l0);
+      const R tdv = madd(t[2], dv2, madd(t[1], dv1, t[0] * dv0));
+      R rate;
+      if constexpr (sizeof(R) == 8) rate = tdv / (e * K.l0);
+      else rate = tdv * (ie * K.il0);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         out.shear[c] = sh[c] * K.l0;
@@ -379,9 +398,15 @@ namespace erod {
         const double f0 = xk2[0] - xk1[0], f1 = xk2[1] - xk1[1], f2 = xk2[2] - xk1[2];
         const R l1 = len_<R>(f0, f1, f2);
         const R D = R(0.5) * (l + l1);
-        const R eps = div_(D, K.D0);
+        R eps;
         R kap[3];
-        curvature<R>(qk, qk1, K.D0, kap);
+        if constexpr (sizeof(R) == 8) {
+          eps = D / K.D0;
+          curvature<R>(qk, qk1, K.D0, kap);
+        } else {
+          eps = D * K.iD0;
+          curvature<R>(qk, qk1, K.iD0, kap);
+        }
         R M[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) M[c] = K.B[c] * (kap[c] - K.kap0[c]);
@@ -699,19 +724,23 @@ namespace erod {
           }
           const R gf = gfj[j], gt = gtj[j];
           // internal force on node k: difference of c, then damping
+          // The rod-end cases of the difference and quadrature operators need no
+          // branches: the -1 neighbour terms are zero at the first element and
+          // the Voronoi terms (P, Rv) are zero at the last one, and x - 0, 0 - x
+          // and 0 + x are exact, as with the reference's zero ghosts.
           R f[3];
           R vk[3] = {static_cast<R>(v[j][0]), static_cast<R>(v[j][1]), static_cast<R>(v[j][2])};
           const R mg = mk * gf;
 #pragma unroll
-          for (int c = 0; c < 3; ++c) f[c] = (first ? md.c[c] : md.c[c] - cm[c]) - mg * vk[c];
+          for (int c = 0; c < 3; ++c) f[c] = (md.c[c] - cm[c]) - mg * vk[c];
           // internal torque on element k
           const R em = R(0.5) * (mk + mkn);
           const R eg = em * gt;
           R tau[3];
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            R d = first ? md.P[c] : (last ? -Pm[c] : md.P[c] - Pm[c]);
-            R qd = first ? R(0.5) * md.Rv[c] : (last ? R(0.5) * Rm[c] : R(0.5) * (md.Rv[c] + Rm[c]));
+            const R d = md.P[c] - Pm[c];
+            const R qd = R(0.5) * (md.Rv[c] + Rm[c]);
             tau[c] = ((((d + qd) + md.shear[c]) + md.lag[c]) + md.unst[c]) - eg * static_cast<R>(w[j][c]);
           }
           // external loads of node k (forcing, then environment)

@@ -167,8 +167,9 @@ def test_fp64_fused_is_independent_of_steps_per_launch():
 
 
 def test_fp32_bulk_launch_segmentation_is_a_rounding_effect():
-    """Merging the half steps inside a launch changes FP32-bulk results only
-    at FP64 rounding level."""
+    """Merging the half steps inside a launch changes x at FP64 rounding
+    level; through the FP32 rounding of node differences that can grow to
+    FP32-bulk's own rounding noise, still 1000x below its accuracy bound."""
     wl = random_workload(10, 4, 120, seed=19, steps=60)
     a = device_block(wl)
     a.set_fused_config(0, 0, 60)
@@ -178,7 +179,7 @@ def test_fp32_bulk_launch_segmentation_is_a_rounding_effect():
     b.step_fused(60, FP32_BULK)
     for r in range(len(wl.n_elems)):
         x, y = a.get_rod("Position", r), b.get_rod("Position", r)
-        assert np.abs(x - y).max() <= 1e-12 * max(1.0, np.abs(y).max()), r
+        assert np.abs(x - y).max() <= 1e-9 * max(1.0, np.abs(y).max()), r
 
 
 def test_fused_graph_replay_equals_direct():
