@@ -260,25 +260,37 @@ namespace erod {
     int rc = fused_prepare(b, precision);
     if (rc) return rc;
     const int snap_fields[4] = {EROD_POSITION, EROD_VELOCITY, EROD_DIRECTOR, EROD_ANGULAR_VELOCITY};
-    std::vector<double*> snap(5, nullptr);
-    std::vector<size_t> bytes(5);
+    // snapshot buffers and the plan settings the trials change, released and
+    // restored on every return
+    struct Guard {
+      erod_batch* b;
+      double* snap[5] = {};
+      int lanes, epl;
+      bool budgets;
+      ~Guard() {
+        for (double* p : snap)
+          if (p) (void)hipFree(p);
+        b->plan.lanes_per_tile = lanes;
+        b->plan.elements_per_thread = epl;
+        b->budgets = budgets;
+        b->last_launches = 0;  // the trial launches are not a fused call's timing
+      }
+    } g{b, {}, b->plan.lanes_per_tile, b->plan.elements_per_thread, b->budgets};
+    size_t bytes[5];
     for (int i = 0; i < 4; ++i) bytes[i] = sizeof(double) * field_info(snap_fields[i]).ncomp * b->field_slots(snap_fields[i]);
     bytes[4] = sizeof(double) * b->n_rods;
     auto dev = [&](int i) { return i < 4 ? b->d_field[snap_fields[i]] : b->d_time; };
     for (int i = 0; i < 5; ++i) {
-      EROD_HIP(hipMalloc(&snap[i], bytes[i]));
-      EROD_HIP(hipMemcpyAsync(snap[i], dev(i), bytes[i], hipMemcpyDeviceToDevice, b->stream));
+      EROD_HIP(hipMalloc(&g.snap[i], bytes[i]));
+      EROD_HIP(hipMemcpyAsync(g.snap[i], dev(i), bytes[i], hipMemcpyDeviceToDevice, b->stream));
     }
     auto restore = [&]() -> int {
-      for (int i = 0; i < 5; ++i) EROD_HIP(hipMemcpyAsync(dev(i), snap[i], bytes[i], hipMemcpyDeviceToDevice, b->stream));
+      for (int i = 0; i < 5; ++i) EROD_HIP(hipMemcpyAsync(dev(i), g.snap[i], bytes[i], hipMemcpyDeviceToDevice, b->stream));
       return EROD_OK;
     };
-    const int keep_lanes = b->plan.lanes_per_tile, keep_epl = b->plan.elements_per_thread;
-    // trial launches leave the step budgets alone
-    const bool keep_budgets = b->budgets;
-    b->budgets = false;
     b->plan.lanes_per_tile = 0;
     b->plan.elements_per_thread = 0;
+    b->budgets = false;  // trial launches leave the step budgets alone
     int best = -1;
     double best_t = 0.0;
     for (int si = 0; si < fused::kNShapes && rc == EROD_OK; ++si) {
@@ -295,20 +307,18 @@ namespace erod {
       }
       if ((rc = restore())) break;
     }
-    b->plan.lanes_per_tile = keep_lanes;
-    b->plan.elements_per_thread = keep_epl;
-    b->budgets = keep_budgets;
+    // the chosen plan is made with the caller's settings and budgets
+    b->plan.lanes_per_tile = g.lanes;
+    b->plan.elements_per_thread = g.epl;
+    b->budgets = g.budgets;
     if (rc == EROD_OK && best >= 0) rc = plan_tiles(b, best);
     if (rc == EROD_OK) rc = restore();
     if (rc == EROD_OK) rc = hip_check(hipStreamSynchronize(b->stream), "autotune");
-    for (double* p : snap)
-      if (p) (void)hipFree(p);
     if (rc == EROD_OK && best < 0) {
       set_error("no feasible fused tile shape");
       rc = EROD_ERR_UNSUPPORTED;
     }
     if (best_ms) *best_ms = best_t / trial_steps;
-    b->last_launches = 0;  // the trial launches are not a fused call's timing
     return rc;
   }
 

@@ -59,6 +59,20 @@ namespace erod {
     explicit DeviceBlock(std::vector<std::uint64_t> n_elems, bool damped = true) : n_(std::move(n_elems)) {
       check(erod_batch_create(n_.data(), n_.size(), damped ? 1 : 0, &b_));
     }
+    // a batch continuing from a checkpoint written by save()
+    static DeviceBlock load(const std::string& path) {
+      erod_batch* b = nullptr;
+      check(erod_load(path.c_str(), &b));
+      std::uint64_t slots = 0;
+      check(erod_batch_slots(b, &slots));
+      std::vector<std::uint64_t> n;
+      std::uint64_t s0 = 0, s1 = 0;
+      for (std::uint64_t r = 0; erod_rod_start(b, r, &s0) == EROD_OK; ++r) {
+        const std::uint64_t next = erod_rod_start(b, r + 1, &s1) == EROD_OK ? s1 : slots + 2;
+        n.push_back(next - s0 - 2);
+      }
+      return DeviceBlock(b, std::move(n));
+    }
     DeviceBlock(const DeviceBlock&) = delete;
     DeviceBlock& operator=(const DeviceBlock&) = delete;
     DeviceBlock(DeviceBlock&& o) noexcept : b_(o.b_), n_(std::move(o.n_)) { o.b_ = nullptr; }
@@ -112,9 +126,55 @@ namespace erod {
     void step_fused(std::uint64_t n, erod_precision p = EROD_FP64) { check(erod_step_fused(b_, p, n)); }
     void synchronize() { check(erod_synchronize(b_)); }
 
+    // slots and budgets (rods of different lengths and horizons in one batch)
+    void set_active_length(std::size_t rod, std::uint64_t n) { check(erod_set_rod_active_length(b_, rod, n)); }
+    void set_step_budget(std::size_t rod, std::uint64_t steps) { check(erod_set_rod_step_budget(b_, rod, steps)); }
+
+    // checkpoints (Block/Serialize.hpp's archive of the block, device side)
+    void save(const std::string& path) { check(erod_save(b_, path.c_str())); }
+
    private:
+    DeviceBlock(erod_batch* b, std::vector<std::uint64_t> n) : b_(b), n_(std::move(n)) {}
     erod_batch* b_ = nullptr;
     std::vector<std::uint64_t> n_;
+  };
+
+  // Continuous batching of requests (erod_server_*): rods of a library batch
+  // are admitted into a pool of slots in size classes and advanced in ticks.
+  class Server {
+   public:
+    Server(DeviceBlock& library, const std::vector<std::pair<std::uint64_t, std::uint64_t>>& classes,
+           erod_precision p = EROD_FP32_BULK, int steps_per_tick = 100, double hbm_budget_bytes = 200e9) {
+      std::vector<std::uint64_t> cap, cnt;
+      for (auto& [c, k] : classes) {
+        cap.push_back(c);
+        cnt.push_back(k);
+      }
+      check(erod_server_create(library.handle(), cap.data(), cnt.data(), int(classes.size()), p, steps_per_tick,
+                               hbm_budget_bytes, &s_));
+    }
+    Server(const Server&) = delete;
+    Server& operator=(const Server&) = delete;
+    ~Server() { erod_server_destroy(s_); }
+
+    void submit(std::uint64_t id, const std::vector<std::uint64_t>& library_rods, std::uint64_t steps) {
+      check(erod_server_submit(s_, id, library_rods.data(), library_rods.size(), steps));
+    }
+    // one tick; the ids of requests that finished
+    std::vector<std::uint64_t> tick() {
+      std::vector<std::uint64_t> done(4096);
+      std::uint64_t n = 0;
+      check(erod_server_tick(s_, done.data(), done.size(), &n));
+      done.resize(n);
+      return done;
+    }
+    void read(std::uint64_t id, std::uint64_t rod, int field, double* host) {
+      check(erod_server_read(s_, id, rod, field, host));
+    }
+    void release(std::uint64_t id) { check(erod_server_release(s_, id)); }
+
+   private:
+    erod_server* s_ = nullptr;
   };
 
   // Straight-rod initializer (the reference's StraightCosseratRodInitializer:

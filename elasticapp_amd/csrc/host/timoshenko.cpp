@@ -4,9 +4,10 @@
 // configuration of examples/timoshenko/TimoshenkoCaseStudy.cpp:27-183 with
 // the rod along x.  Steps with the reference-order stepper over the
 // op-by-op kernels (--plain) or with the fused FP64 kernels, and prints the
-// tip position.
+// tip position.  --resume PATH steps the first half with the fused kernels,
+// writes a checkpoint to PATH, loads it into a new block and steps the rest.
 //
-//   timoshenko [steps] [--plain]
+//   timoshenko [steps] [--plain | --resume PATH]
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -17,6 +18,7 @@
 int main(int argc, char** argv) {
   const std::uint64_t steps = argc > 1 ? std::strtoull(argv[1], nullptr, 10) : 1000;
   const bool plain = argc > 2 && std::strcmp(argv[2], "--plain") == 0;
+  const char* resume = argc > 3 && std::strcmp(argv[2], "--resume") == 0 ? argv[3] : nullptr;
   try {
     erod::StraightRod p{100, 0.25, 3.0, {0, 0, 0}, {1, 0, 0}, {0, 0, 1}, 5000.0, 1e6, 1e6 / 1.99, 0.1, 0.1};
     std::vector<erod::DeviceBlock> blocks;
@@ -35,6 +37,14 @@ int main(int argc, char** argv) {
     double t = 0.0;
     if (plain) {
       for (std::uint64_t i = 0; i < steps; ++i) erod::PositionVerlet::step(blocks, t, dt);
+    } else if (resume) {
+      blocks[0].set_dt(dt);
+      blocks[0].step_fused(steps / 2, EROD_FP64);
+      blocks[0].save(resume);
+      blocks.clear();
+      blocks.push_back(erod::DeviceBlock::load(resume));
+      blocks[0].step_fused(steps - steps / 2, EROD_FP64);
+      t = double(steps) * dt;
     } else {
       blocks[0].set_dt(dt);
       blocks[0].step_fused(steps, EROD_FP64);

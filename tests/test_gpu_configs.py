@@ -67,3 +67,26 @@ def test_uniform_batch_against_oracle():
         for t in STATE:
             a, b = ops.get_rod(t, r), f500.get_rod(t, r)
             assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max(), (r, t)
+
+
+def _driver(*args):
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "elasticapp_amd", "timoshenko")
+    out = subprocess.run([exe] + [str(a) for a in args], capture_output=True, text=True, timeout=120, check=True).stdout
+    w = out.split()
+    return np.array([float(v) for v in w[w.index("tip") + 1:w.index("tip") + 4]])
+
+
+def test_cpp_driver_matches_oracle(tmp_path):
+    """The C++ host layer (erod.hpp): the Timoshenko driver's tip after 2,000
+    steps on the op-by-op and fused paths, and resumed from a checkpoint
+    halfway, against the oracle on the same configuration."""
+    wl = timoshenko_c0(steps=2000)
+    sysm = oracle_system(wl, max_block_elems=1024)
+    sysm.step(wl.dt, wl.steps)
+    tip = sysm.rod_field(0, "Position")[:, -1]
+    fused, plain = _driver(2000), _driver(2000, "--plain")
+    resumed = _driver(2000, "--resume", tmp_path / "t.ckpt")
+    assert np.array_equal(fused, resumed)
+    for got in (fused, plain):
+        assert np.abs(got - tip).max() <= 1e-12 * np.abs(tip).max(), (got, tip)
