@@ -101,7 +101,7 @@ def cpu_baseline(wl, budget_s=12.0):
     one std::thread per host core, timed over enough steps to fill ~budget_s."""
     from elasticapp_amd.workloads import Workload
     from tests.gpu_helpers import oracle_system
-    threads = len(os.sched_getaffinity(0))
+    threads = min(16, len(os.sched_getaffinity(0)))  # the host share of one GPU
     n_rods = min(512, len(wl.n_elems))
     sub = Workload("cpu_sample", wl.n_elems[:n_rods], wl.states[:n_rods], wl.dt, 0, forces=wl.forces[:n_rods])
     sysm = oracle_system(sub, max_block_elems=1024)

@@ -23,6 +23,7 @@
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <string>
 #include <unordered_map>
 #include <vector>
 
@@ -305,10 +306,25 @@ int erod_server_tick(erod_server* s, uint64_t* done_ids, uint64_t capacity, uint
   }
   int rc = EROD_OK;
   if (!dst.empty()) {
-    if ((rc = erod_copy_rods(p, dst.data(), s->library, src.data(), dst.size()))) return rc;
-    for (erod_server::Request* rq : admitted) {
+    rc = erod_copy_rods(p, dst.data(), s->library, src.data(), dst.size());
+    for (erod_server::Request* rq : admitted)
       for (uint64_t slot : rq->slots)
-        if ((rc = erod_set_rod_step_budget(p, slot, rq->steps))) return rc;
+        if (rc == EROD_OK) rc = erod_set_rod_step_budget(p, slot, rq->steps);
+    if (rc != EROD_OK) {
+      // put the requests back at the head of the queue, in order, and free their slots
+      const std::string msg = erod_last_error();
+      for (auto it = admitted.rbegin(); it != admitted.rend(); ++it) {
+        for (uint64_t slot : (*it)->slots) {
+          empty_slot(p, slot);
+          give_back(s, slot);
+        }
+        (*it)->slots.clear();
+        s->queue.push_front(*it);
+      }
+      set_error(msg);
+      return rc;
+    }
+    for (erod_server::Request* rq : admitted) {
       rq->running = true;
       s->running.push_back(rq);
       s->active_elements += rq->elements;
