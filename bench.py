@@ -164,7 +164,9 @@ def c4_leg(args, torch):
     blk.close()
     vp = elems * args.c4_plain_steps / elp
     log("c4 contact plain fp64: %.3e element-steps/s" % vp)
-    return {"workload": "contact_c4: %d rods x %d elements, L 1 m, r 0.01 m, rho 1000, E 1e6, rest curvature "
+    cpu = None if args.no_cpu else _cpu_rate(wl, 1024, 5.0, min(16, len(os.sched_getaffinity(0))))
+    return {"cpu_baseline": cpu,
+            "workload": "contact_c4: %d rods x %d elements, L 1 m, r 0.01 m, rho 1000, E 1e6, rest curvature "
                         "N(0,0.5), gravity -9.81 z, plane z=0 k 1e3 c 10 mu 0.3 ct 10" % (len(n), n[0]),
             "elements": elems, "dt": wl.dt, "steps": args.c4_steps, "value": v, "unit": "element-steps/s",
             "ms_per_step": el * 1e3 / args.c4_steps, "tile_lanes": lanes, "elements_per_thread": epl,
@@ -173,6 +175,78 @@ def c4_leg(args, torch):
                          "frac": achieved / HBM_PEAK_GBS, "kernel_ms_per_launch": kms, "launches": launches,
                          "algorithmic_bytes_per_launch": bytes_launch},
             "fp64_plain": {"value": vp, "steps": args.c4_plain_steps}, "ratio_fp32bulk_vs_plain_fp64": v / vp}
+
+
+def _cpu_rate(wl, n_rods, budget_s, threads):
+    """Oracle element-steps/s on the first n_rods of a workload, blocks of <=
+    1024 elements, about budget_s seconds of stepping."""
+    from elasticapp_amd.workloads import Workload
+    from tests.gpu_helpers import oracle_system
+    sub = Workload("cpu_sample", wl.n_elems[:n_rods], wl.states[:n_rods], wl.dt, 0,
+                   forces=wl.forces[:n_rods] if wl.forces else [], clamps=wl.clamps[:n_rods] if wl.clamps else [],
+                   gravity=wl.gravity, contact=wl.contact)
+    sysm = oracle_system(sub, max_block_elems=1024)
+    elems = sum(sub.n_elems)
+    t = time.perf_counter()
+    sysm.step(wl.dt, 5, threads)
+    probe = time.perf_counter() - t
+    steps = int(max(10, min(50000, budget_s / max(probe / 5, 1e-7))))
+    t = time.perf_counter()
+    sysm.step(wl.dt, steps, threads)
+    el = time.perf_counter() - t
+    return {"value": elems * steps / el, "unit": "element-steps/s", "cores": threads, "kind": "port",
+            "sample": "%d rods (%d elements), %d FP64 steps, %.1f s" % (n_rods, elems, steps, el)}
+
+
+def small_configs_leg(args, torch, cpu):
+    """The Timoshenko cantilever (1 rod x 100, 50,000 steps) and the uniform
+    batch (1024 rods x 64, 10,000 steps): FP64 fused and plain rates, the
+    fused FP64 error against the oracle for the cantilever, the roofline
+    fraction of the uniform batch, and the oracle timed beside them."""
+    from elasticapp_amd import FP64
+    from elasticapp_amd.workloads import build_block, timoshenko_c0, uniform_c1
+    from tests.gpu_helpers import oracle_system, rel_state_error
+    threads = min(16, len(os.sched_getaffinity(0)))
+    out = {}
+    wl = timoshenko_c0(steps=50000)
+    blk = build_block(wl)
+    sync = lambda: (blk.synchronize(), torch.cuda.synchronize())
+    blk.step_fused(1000, FP64)
+    blk = build_block(wl)
+    el = timed(lambda: blk.step_fused(wl.steps, FP64), sync)
+    sysm = oracle_system(wl, max_block_elems=1024)
+    t = time.perf_counter()
+    sysm.step(wl.dt, wl.steps, 1)
+    cpu_el = time.perf_counter() - t
+    err = rel_state_error(blk, sysm)
+    plain = build_block(wl)
+    elp = timed(lambda: plain.step(wl.dt, 1000), lambda: (plain.synchronize(), torch.cuda.synchronize()))
+    out["c0_timoshenko"] = {"steps": wl.steps, "fp64_fused": 100 * wl.steps / el, "fp64_plain": 100 * 1000 / elp,
+                            "unit": "element-steps/s", "max_rel_err_vs_oracle": max(err.values()),
+                            "tip": blk.get_rod("Position", 0)[:, -1].tolist(),
+                            "cpu_baseline": {"value": 100 * wl.steps / cpu_el, "unit": "element-steps/s", "cores": 1,
+                                             "kind": "port", "sample": "the whole run, 1 thread"}}
+    log("c0 timoshenko: fp64 fused %.3e, plain %.3e element-steps/s, max rel err vs oracle %.1e"
+        % (out["c0_timoshenko"]["fp64_fused"], out["c0_timoshenko"]["fp64_plain"], max(err.values())))
+    wl = uniform_c1(steps=10000)
+    n = np.array(wl.n_elems)
+    blk = build_block(wl)
+    blk.set_fused_config(0, 0, 100)
+    blk.autotune(FP64, 100)
+    sync = lambda: (blk.synchronize(), torch.cuda.synchronize())
+    blk.step_fused_graph(100, 100, FP64)
+    sync()
+    el = timed(lambda: blk.step_fused_graph(wl.steps, 100, FP64), sync)
+    kms, launches = blk.last_kernel_ms()
+    achieved = float(b64(n).sum()) * (wl.steps // launches) / (kms * 1e-3) / 1e9
+    elems = int(n.sum())
+    out["c1_uniform"] = {"steps": wl.steps, "fp64_fused": elems * wl.steps / el, "unit": "element-steps/s",
+                         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                      "frac": achieved / HBM_PEAK_GBS, "kernel_ms_per_launch": kms,
+                                      "algorithmic_bytes_per_launch": float(b64(n).sum()) * (wl.steps // launches)},
+                         "cpu_baseline": None if not cpu else _cpu_rate(wl, 1024, 5.0, threads)}
+    log("c1 uniform: fp64 fused %.3e element-steps/s, %.0f GB/s algorithmic" % (out["c1_uniform"]["fp64_fused"], achieved))
+    return out
 
 
 def c3_leg(args, torch, capacity):
@@ -241,6 +315,7 @@ def main():
     ap.add_argument("--c4-steps", type=int, default=2000, help="steps of the contact batch (0: skip it)")
     ap.add_argument("--c4-plain-steps", type=int, default=20)
     ap.add_argument("--c3-requests", type=int, default=400, help="requests of the serving leg (0: skip it)")
+    ap.add_argument("--small-configs", type=int, default=1, help="1: also run the cantilever and the uniform batch")
     ap.add_argument("--tile", default="", help="LANES,EPL: fixed tile shape instead of the autotuner's choice")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rods", type=int, default=4096)
@@ -350,6 +425,7 @@ def main():
     log("plain fp64: %.3f s for %d steps -> %.3e element-steps/s" % (elp, args.plain_steps, vp))
     blkp.close()
 
+    small = small_configs_leg(args, torch, not args.no_cpu) if args.small_configs else None
     c4 = c4_leg(args, torch) if args.c4_steps > 0 else None
     c3 = c3_leg(args, torch, v32) if args.c3_requests > 0 else None
 
@@ -388,6 +464,7 @@ def main():
                                "algorithmic_bytes_per_launch": bytes_s,
                                "element_steps_per_s": elems / (kms_s * 1e-3)},
         "stream_copy_gbs": copy_gbs,
+        "c0_c1": small,
         "c4_contact": c4,
         "c3_serving": c3,
         "cpu_baseline": cpu,
