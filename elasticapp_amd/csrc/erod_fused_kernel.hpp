@@ -84,6 +84,9 @@ namespace erod {
       else return sqrt_(static_cast<float>(__builtin_fma(d2, d2, __builtin_fma(d1, d1, d0 * d0))));
     }
 
+    // FP32 length of an FP32 difference
+    __device__ __forceinline__ float len32(float a, float b, float c) { return sqrt_(madd(c, c, madd(b, b, a * a))); }
+
     template <typename R>
     __device__ __forceinline__ void cross3(const R a[3], const R b[3], R o[3]) {
       o[0] = madd(a[1], b[2], -(a[2] * b[1]));
@@ -262,50 +265,6 @@ namespace erod {
       }
     }
 
-    // the constants of a thread's P elements, one field line at a time: the P
-    // loads of a line fall in the same cache lines and are issued back to back
-    template <typename R, int P>
-    __device__ __forceinline__ void load_consts(const Consts<R>& C, const uint64_t (&s)[P], uint64_t E, const bool (&ok)[P],
-                                                const bool (&has_vor)[P], ElemConst<R> (&K)[P]) {
-      const uint64_t V = E - 1;
-#pragma unroll
-      for (int j = 0; j < P; ++j)
-        if (ok[j]) K[j].m = C.m[s[j]];
-#pragma unroll
-      for (int j = 0; j < P; ++j)
-        if (ok[j]) K[j].mn = C.m[s[j] + 1];
-#pragma unroll
-      for (int j = 0; j < P; ++j)
-        if (ok[j]) K[j].invm = C.invm[s[j]];
-#pragma unroll
-      for (int j = 0; j < P; ++j)
-        if (ok[j]) K[j].l0 = C.l0[s[j]];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-#pragma unroll
-        for (int j = 0; j < P; ++j)
-          if (ok[j]) K[j].sig0[c] = C.sig0[c * E + s[j]];
-#pragma unroll
-        for (int j = 0; j < P; ++j)
-          if (ok[j]) K[j].S[c] = C.S[c * E + s[j]];
-#pragma unroll
-        for (int j = 0; j < P; ++j)
-          if (ok[j]) K[j].J[c] = C.J[c * E + s[j]];
-#pragma unroll
-        for (int j = 0; j < P; ++j)
-          if (ok[j]) K[j].iJ[c] = C.iJ[c * E + s[j]];
-      }
-#pragma unroll
-      for (int j = 0; j < P; ++j) K[j].D0 = ok[j] && has_vor[j] ? C.D0[s[j]] : R(1);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-#pragma unroll
-        for (int j = 0; j < P; ++j) K[j].kap0[c] = ok[j] && has_vor[j] ? C.kap0[c * V + s[j]] : R(0);
-#pragma unroll
-        for (int j = 0; j < P; ++j) K[j].B[c] = ok[j] && has_vor[j] ? C.B[c * V + s[j]] : R(0);
-      }
-    }
-
     // per-element values carried from the first to the second half of a step
     template <typename R>
     struct Mid {
@@ -323,22 +282,30 @@ namespace erod {
     // (the FP64 state for R = double, its FP32 copy for R = float).
     template <typename R>
     __device__ __forceinline__ void element_forces(const double xk[3], const double xk1[3], const double xk2[3],
-                                                   const double vk[3], const double vk1[3], const double qk[9],
-                                                   const R qr[9], const double qk1[9], const double wk[3], bool has_vor,
-                                                   const ElemConst<R>& K, Mid<R>& out) {
-      // spanwise geometry: node difference in FP64, then R
-      const double d0 = xk1[0] - xk[0], d1 = xk1[1] - xk[1], d2 = xk1[2] - xk[2];
-      const R l = len_<R>(d0, d1, d2);
-      R t[3];
+                                                   const float dn[3], const double vk[3], const double vk1[3],
+                                                   const double qk[9], const R qr[9], const double qk1[9],
+                                                   const double wk[3], bool has_vor, const ElemConst<R>& K,
+                                                   Mid<R>& out) {
+      // spanwise geometry.  FP64: the reference's node difference, length and
+      // tangent.  FP32-bulk: the node difference in FP64 rounded to FP32, then
+      // everything in FP32; an element's length is a function of that rounded
+      // difference alone, so the neighbour that needs it (for the Voronoi
+      // length) gets the same bits from the same difference (dn).
+      R l, t[3];
       if constexpr (sizeof(R) == 8) {
+        const double d0 = xk1[0] - xk[0], d1 = xk1[1] - xk[1], d2 = xk1[2] - xk[2];
+        l = len_<R>(d0, d1, d2);
         t[0] = d0 / l;
         t[1] = d1 / l;
         t[2] = d2 / l;
       } else {
+        const float d0 = static_cast<float>(xk1[0] - xk[0]), d1 = static_cast<float>(xk1[1] - xk[1]),
+                    d2 = static_cast<float>(xk1[2] - xk[2]);
+        l = len32(d0, d1, d2);
         const float il = rcp_(l);
-        t[0] = static_cast<float>(d0) * il;
-        t[1] = static_cast<float>(d1) * il;
-        t[2] = static_cast<float>(d2) * il;
+        t[0] = d0 * il;
+        t[1] = d1 * il;
+        t[2] = d2 * il;
       }
       R e, ie;
       if constexpr (sizeof(R) == 8) {
@@ -395,8 +362,13 @@ namespace erod {
       }
       // Voronoi k
       if (has_vor) {
-        const double f0 = xk2[0] - xk1[0], f1 = xk2[1] - xk1[1], f2 = xk2[2] - xk1[2];
-        const R l1 = len_<R>(f0, f1, f2);
+        R l1;
+        if constexpr (sizeof(R) == 8) {
+          const double f0 = xk2[0] - xk1[0], f1 = xk2[1] - xk1[1], f2 = xk2[2] - xk1[2];
+          l1 = len_<R>(f0, f1, f2);
+        } else {
+          l1 = len32(dn[0], dn[1], dn[2]);
+        }
         const R D = R(0.5) * (l + l1);
         R eps;
         R kap[3];
@@ -451,7 +423,7 @@ namespace erod {
 
     // ---- the kernel ------------------------------------------------------------------
 
-    template <typename R, int P, int NT>
+    template <typename R, int P, int NT, bool kEnv>
     __global__ void __launch_bounds__(NT) k_fused(double* __restrict__ X, double* __restrict__ Vel, double* __restrict__ Qd,
                                                   double* __restrict__ W, Consts<R> C, const RodInfo* __restrict__ rods,
                                                   const Tile* __restrict__ tiles, const uint32_t* __restrict__ tile_rods,
@@ -465,10 +437,20 @@ namespace erod {
       const Tile tile = tiles[blockIdx.x];
       const int tid = threadIdx.x;
 
-      // exchange 1: per thread, its first element's node x, v, director and the next node
-      __shared__ double s1[18][NT];
+      // exchange 1: per thread, its first element's node x, v and director, and
+      // the thread's second node (FP64), or the FP32 difference of its second
+      // and first node (FP32-bulk; for P = 1 only when the element is the rod's
+      // last, the second node being the rod's tail)
+      constexpr int kS1 = kCache ? 15 : 18;
+      __shared__ double s1[kS1][NT];
+      __shared__ float s5[kCache ? 3 : 1][NT];
       // exchange 2: per thread, its last element's c, P, R
       __shared__ R s3[9][NT];
+      // FP32-bulk: 18 constants of every lane of the tile, for the whole launch:
+      //   0 m, 1 l0, 2-4 sig0, 5-7 S, 8-10 J, 11 D0, 12-14 kap0, 15-17 B
+      // (1/m, 1/J, 1/l0, 1/D0 are formed when used; the defaults of an element
+      // without a Voronoi region are stored for it)
+      __shared__ float sc[kCache ? 18 : 1][kCache ? P * NT : 1];
 
       // ---- per element: rod, local index, slots, state -----------------------
       int rodi[P];
@@ -480,7 +462,6 @@ namespace erod {
       uint32_t rflags[P];
       uint32_t lim[P];  // steps this launch may take for the element's rod (its step budget)
       R gfj[P], gtj[P];
-      ElemConst<R> kc[kCache ? P : 1];
       R tail_m = R(0), tail_im = R(0);
       double xt[3] = {0, 0, 0}, vt[3] = {0, 0, 0};
       int tailj = -1;
@@ -546,12 +527,38 @@ namespace erod {
         }
       }
       if constexpr (kCache) {
-        bool has_vor[P];
+        // lane el = tid + i NT: consecutive lanes of a rod are consecutive slots
+        const uint64_t V = E - 1;
 #pragma unroll
-        for (int j = 0; j < P; ++j) has_vor[j] = kk[j] < nn[j] - 1;
+        for (int i = 0; i < P; ++i) {
+          const uint32_t el = uint32_t(tid) + uint32_t(i) * NT;
+          if (el >= tile.lanes) continue;
+          uint32_t lo = 0, hi = tile.count;
+          while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (tile_lane0[tile.first + mid] <= el) lo = mid;
+            else hi = mid;
+          }
+          const RodInfo& ri = rods[tile_rods[tile.first + lo]];
+          const uint32_t k = el - tile_lane0[tile.first + lo];
+          const uint64_t sl = ri.start + k;
+          sc[0][el] = C.m[sl];
+          sc[1][el] = C.l0[sl];
 #pragma unroll
-        for (int j = 0; j < P; ++j)
-          if (valid[j]) load_const<R>(C, slot[j], E, has_vor[j], kc[kCache ? j : 0]);
+          for (int c = 0; c < 3; ++c) {
+            sc[2 + c][el] = C.sig0[c * E + sl];
+            sc[5 + c][el] = C.S[c * E + sl];
+            sc[8 + c][el] = C.J[c * E + sl];
+          }
+          const bool vor = k + 1 < ri.n;
+          sc[11][el] = vor ? C.D0[sl] : 1.0f;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            sc[12 + c][el] = vor ? C.kap0[c * V + sl] : 0.0f;
+            sc[15 + c][el] = vor ? C.B[c * V + sl] : 0.0f;
+          }
+        }
+        __syncthreads();
       }
 
       for (int step = 0; step < nsteps; ++step) {
@@ -591,7 +598,9 @@ namespace erod {
           for (int c = 0; c < 3; ++c) {
             s1[c][tid] = x[0][c];
             s1[3 + c][tid] = v[0][c];
-            s1[15 + c][tid] = last0 ? xt[c] : (P > 1 ? x[P > 1 ? 1 : 0][c] : 0.0);
+            const double xs = last0 ? xt[c] : (P > 1 ? x[P > 1 ? 1 : 0][c] : 0.0);
+            if constexpr (!kCache) s1[kCache ? 0 : 15 + c][tid] = xs;
+            else s5[kCache ? c : 0][tid] = static_cast<float>(xs - x[0][c]);
           }
 #pragma unroll
           for (int m = 0; m < 9; ++m) s1[6 + m][tid] = q[0][m];
@@ -611,6 +620,7 @@ namespace erod {
           }
           const bool last = kk[j] == nn[j] - 1;
           double xk1[3], xk2[3] = {0, 0, 0}, vk1[3], qk1[9];
+          float dn[3] = {0.0f, 0.0f, 0.0f};
           if (last) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -640,38 +650,61 @@ namespace erod {
               for (int m = 0; m < 9; ++m) qk1[m] = s1[6 + m][tn];
             }
             const bool next_last = kk[j] + 1 == nn[j] - 1;
-            if (next_last) {
-              if (j + 1 < P) {
+            if (kCache && j == P - 1 && (P > 1 || next_last)) {
+              // FP32-bulk: the next element's rounded difference, from its thread
 #pragma unroll
-                for (int c = 0; c < 3; ++c) xk2[c] = xt[c];
+              for (int c = 0; c < 3; ++c) dn[c] = s5[kCache ? c : 0][tn];
+            } else {
+              if (next_last) {
+                if (j + 1 < P) {
+#pragma unroll
+                  for (int c = 0; c < 3; ++c) xk2[c] = xt[c];
+                } else {
+#pragma unroll
+                  for (int c = 0; c < 3; ++c) xk2[c] = s1[kCache ? 0 : 15 + c][tn];
+                }
+              } else if (j + 2 < P) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) xk2[c] = x[j + 2 < P ? j + 2 : j][c];
+              } else if (j + 2 == P) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) xk2[c] = s1[c][tn];
+              } else if (P > 1) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) xk2[c] = s1[kCache ? 0 : 15 + c][tn];
               } else {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) xk2[c] = s1[15 + c][tn];
+                for (int c = 0; c < 3; ++c) xk2[c] = s1[c][tn2];
               }
-            } else if (j + 2 < P) {
 #pragma unroll
-              for (int c = 0; c < 3; ++c) xk2[c] = x[j + 2 < P ? j + 2 : j][c];
-            } else if (j + 2 == P) {
-#pragma unroll
-              for (int c = 0; c < 3; ++c) xk2[c] = s1[c][tn];
-            } else if (P > 1) {
-#pragma unroll
-              for (int c = 0; c < 3; ++c) xk2[c] = s1[15 + c][tn];
-            } else {
-#pragma unroll
-              for (int c = 0; c < 3; ++c) xk2[c] = s1[c][tn2];
+              for (int c = 0; c < 3; ++c) dn[c] = static_cast<float>(xk2[c] - xk1[c]);
             }
           } else {
 #pragma unroll
             for (int m = 0; m < 9; ++m) qk1[m] = 0.0;
           }
           ElemConst<R> K;
-          if constexpr (kCache) K = kc[kCache ? j : 0];
-          else load_const<R>(C, slot[j], E, has_vor, K);
+          if constexpr (kCache) {
+            const uint32_t el = uint32_t(tid) * P + j;
+            K.l0 = sc[1][el];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              K.sig0[c] = sc[kCache ? 2 + c : 0][kCache ? el : 0];
+              K.S[c] = sc[kCache ? 5 + c : 0][kCache ? el : 0];
+              K.J[c] = sc[kCache ? 8 + c : 0][kCache ? el : 0];
+              K.kap0[c] = sc[kCache ? 12 + c : 0][kCache ? el : 0];
+              K.B[c] = sc[kCache ? 15 + c : 0][kCache ? el : 0];
+            }
+            K.D0 = sc[kCache ? 11 : 0][kCache ? el : 0];
+            K.il0 = rcp_(K.l0);
+            K.iD0 = rcp_(K.D0);
+          } else {
+            load_const<R>(C, slot[j], E, has_vor, K);
+          }
           R qr[9];
 #pragma unroll
           for (int m = 0; m < 9; ++m) qr[m] = static_cast<R>(q[j][m]);
-          element_forces<R>(x[j], xk1, xk2, v[j], vk1, q[j], qr, qk1, w[j], has_vor, K, mid[j]);
+          element_forces<R>(x[j], xk1, xk2, dn, v[j], vk1, q[j], qr, qk1, w[j], has_vor, K, mid[j]);
         }
         // ---- exchange 2 -----------------------------------------------------------
 #pragma unroll
@@ -709,12 +742,12 @@ namespace erod {
           const Mid<R>& md = mid[j];
           R mk, mkn, im, iJ[3];
           if constexpr (kCache) {
-            const ElemConst<R>& K = kc[kCache ? j : 0];
-            mk = K.m;
-            mkn = K.mn;
-            im = K.invm;
+            const uint32_t el = uint32_t(tid) * P + j;
+            mk = sc[0][el];
+            mkn = last ? tail_m : sc[0][kCache ? el + 1 : 0];
+            im = rcp_(mk);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) iJ[c] = K.iJ[c];
+            for (int c = 0; c < 3; ++c) iJ[c] = rcp_(sc[kCache ? 8 + c : 0][kCache ? el : 0]);
           } else {
             mk = C.m[s];
             mkn = C.m[s + 1];
@@ -745,7 +778,7 @@ namespace erod {
           }
           // external loads of node k (forcing, then environment)
           double fe[3] = {0.0, 0.0, 0.0};
-          node_environment(env, static_cast<double>(mk), x[j], v[j], fe);
+          if constexpr (kEnv) node_environment(env, static_cast<double>(mk), x[j], v[j], fe);
           const double dtt = dt[j];
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
@@ -767,7 +800,7 @@ namespace erod {
 #pragma unroll
               for (int c = 0; c < 3; ++c) fen[c] = ri.force[c] * rv;
             }
-            node_environment(env, static_cast<double>(tail_m), xt, vt, fen);
+            if constexpr (kEnv) node_environment(env, static_cast<double>(tail_m), xt, vt, fen);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
               const R fn = -md.c[c] - mgn * static_cast<R>(vt[c]);
@@ -845,13 +878,23 @@ namespace erod {
       }
     }
 
+    // batches without gravity or contact run a kernel without environment code
+    // (and without the registers its parameters would hold)
     template <typename R, int P, int NT>
     inline int launch_tiles(erod_batch* b, const Consts<R>& C, int steps) {
       if (b->plan.n_tiles == 0) return EROD_OK;
-      hipLaunchKernelGGL((k_fused<R, P, NT>), dim3(static_cast<unsigned>(b->plan.n_tiles)), dim3(NT), 0, b->stream,
-                         b->d_field[EROD_POSITION], b->d_field[EROD_VELOCITY], b->d_field[EROD_DIRECTOR],
-                         b->d_field[EROD_ANGULAR_VELOCITY], C, b->d_rods, b->plan.d_tiles, b->plan.d_tile_rods,
-                         b->plan.d_tile_lane0, b->d_time, b->budgets ? b->d_steps_left : nullptr, b->env, steps, b->E);
+      const dim3 grid(static_cast<unsigned>(b->plan.n_tiles)), block(NT);
+      uint32_t* budgets = b->budgets ? b->d_steps_left : nullptr;
+      if (b->env.gravity || b->env.contact)
+        hipLaunchKernelGGL((k_fused<R, P, NT, true>), grid, block, 0, b->stream, b->d_field[EROD_POSITION],
+                           b->d_field[EROD_VELOCITY], b->d_field[EROD_DIRECTOR], b->d_field[EROD_ANGULAR_VELOCITY], C,
+                           b->d_rods, b->plan.d_tiles, b->plan.d_tile_rods, b->plan.d_tile_lane0, b->d_time, budgets,
+                           b->env, steps, b->E);
+      else
+        hipLaunchKernelGGL((k_fused<R, P, NT, false>), grid, block, 0, b->stream, b->d_field[EROD_POSITION],
+                           b->d_field[EROD_VELOCITY], b->d_field[EROD_DIRECTOR], b->d_field[EROD_ANGULAR_VELOCITY], C,
+                           b->d_rods, b->plan.d_tiles, b->plan.d_tile_rods, b->plan.d_tile_lane0, b->d_time, budgets,
+                           b->env, steps, b->E);
       return hip_check(hipGetLastError(), "k_fused");
     }
 
