@@ -449,8 +449,11 @@ namespace erod {
       // FP32-bulk: 18 constants of every lane of the tile, for the whole launch:
       //   0 m, 1 l0, 2-4 sig0, 5-7 S, 8-10 J, 11 D0, 12-14 kap0, 15-17 B
       // (1/m, 1/J, 1/l0, 1/D0 are formed when used; the defaults of an element
-      // without a Voronoi region are stored for it)
+      // without a Voronoi region are stored for it).  Lane el = t P + j is kept
+      // at j NT + t, so that a wave reading element j of its threads reads
+      // consecutive words.
       __shared__ float sc[kCache ? 18 : 1][kCache ? P * NT : 1];
+      auto cidx = [](uint32_t el) { return (el % P) * NT + el / P; };
 
       // ---- per element: rod, local index, slots, state -----------------------
       int rodi[P];
@@ -542,20 +545,21 @@ namespace erod {
           const RodInfo& ri = rods[tile_rods[tile.first + lo]];
           const uint32_t k = el - tile_lane0[tile.first + lo];
           const uint64_t sl = ri.start + k;
-          sc[0][el] = C.m[sl];
-          sc[1][el] = C.l0[sl];
+          const uint32_t ce = cidx(el);
+          sc[0][ce] = C.m[sl];
+          sc[1][ce] = C.l0[sl];
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            sc[2 + c][el] = C.sig0[c * E + sl];
-            sc[5 + c][el] = C.S[c * E + sl];
-            sc[8 + c][el] = C.J[c * E + sl];
+            sc[2 + c][ce] = C.sig0[c * E + sl];
+            sc[5 + c][ce] = C.S[c * E + sl];
+            sc[8 + c][ce] = C.J[c * E + sl];
           }
           const bool vor = k + 1 < ri.n;
-          sc[11][el] = vor ? C.D0[sl] : 1.0f;
+          sc[11][ce] = vor ? C.D0[sl] : 1.0f;
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            sc[12 + c][el] = vor ? C.kap0[c * V + sl] : 0.0f;
-            sc[15 + c][el] = vor ? C.B[c * V + sl] : 0.0f;
+            sc[12 + c][ce] = vor ? C.kap0[c * V + sl] : 0.0f;
+            sc[15 + c][ce] = vor ? C.B[c * V + sl] : 0.0f;
           }
         }
         __syncthreads();
@@ -685,7 +689,7 @@ namespace erod {
           }
           ElemConst<R> K;
           if constexpr (kCache) {
-            const uint32_t el = uint32_t(tid) * P + j;
+            const uint32_t el = uint32_t(j) * NT + tid;  // cidx(tid P + j)
             K.l0 = sc[1][el];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -742,9 +746,10 @@ namespace erod {
           const Mid<R>& md = mid[j];
           R mk, mkn, im, iJ[3];
           if constexpr (kCache) {
-            const uint32_t el = uint32_t(tid) * P + j;
+            const uint32_t el = uint32_t(j) * NT + tid;                   // cidx(tid P + j)
+            const uint32_t eln = j + 1 < P ? el + NT : uint32_t(tid) + 1;  // cidx(tid P + j + 1)
             mk = sc[0][el];
-            mkn = last ? tail_m : sc[0][kCache ? el + 1 : 0];
+            mkn = last ? tail_m : sc[0][kCache ? eln : 0];
             im = rcp_(mk);
 #pragma unroll
             for (int c = 0; c < 3; ++c) iJ[c] = rcp_(sc[kCache ? 8 + c : 0][kCache ? el : 0]);
