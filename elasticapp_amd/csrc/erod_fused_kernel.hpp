@@ -4,11 +4,11 @@
 // One workgroup owns a tile of whole rods.  Each thread holds P contiguous
 // elements of the tile (element k, node k and, for the last element of a rod,
 // the rod's last node n) with the state x, v, Q, w in FP64 registers across
-// `nsteps` steps.  The only cross-thread data of a step are the +1/+2
-// neighbour node values, the +1 neighbour director (first exchange) and the
-// -1 neighbour element force / couple terms (second exchange), passed through
-// two LDS arrays, so a step costs two workgroup barriers and no HBM traffic
-// for the state.  P > 1 gives each thread several independent elements (ILP)
+// `nsteps` steps.  The only cross-thread data of a step are the +1 neighbour
+// node values and director and the next element's node difference (first
+// exchange) and the -1 neighbour element force / couple terms (second
+// exchange), passed through LDS, so a step costs two workgroup barriers and
+// no HBM traffic.  P > 1 gives each thread several independent elements (ILP)
 // and lets rods of up to P * NT elements share one workgroup.
 //
 // Arithmetic of one step follows the reference block timestep
@@ -33,8 +33,12 @@
 //   cancellation-prone parts (node differences and the relative rotation of
 //   neighbouring directors) formed in FP64 before rounding.  The log map is
 //   the reference's acos(cos(theta) - 1e-10) rewritten without cancellation
-//   as theta = 2 asin(sqrt(sin^2(theta/2) + 5e-11)).  The constants are
-//   held in FP32 registers for the whole launch.
+//   as theta = 2 asin(sqrt(sin^2(theta/2) + 5e-11)); element lengths come
+//   from node differences rounded to FP32, which the neighbour that needs the
+//   same length receives instead of recomputing it.  The FP32 constants are
+//   held in an LDS table for the whole launch (1/m, 1/J, 1/l0, 1/D0 formed
+//   when used), and inside a launch the closing and opening half steps of
+//   consecutive steps are one kinematic update of dt.
 #pragma once
 
 #include "erod_common.hpp"
