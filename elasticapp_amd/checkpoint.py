@@ -25,7 +25,10 @@ def read_header(path):
         version, damped, n_rods, _ = struct.unpack("<IIQQ", f.read(24))
         if version != 1:
             raise ValueError("unsupported checkpoint version %d" % version)
-        n_elems = np.frombuffer(f.read(8 * n_rods), dtype="<u8")
+        raw = f.read(8 * n_rods)
+        if len(raw) != 8 * n_rods:
+            raise ValueError("truncated checkpoint header: %s" % path)
+        n_elems = np.frombuffer(raw, dtype="<u8")
     return bool(damped), [int(x) for x in n_elems]
 
 

@@ -102,6 +102,14 @@ int erod_load(const char* path, erod_batch** out) {
   if (!get_n(f, magic, 8) || std::memcmp(magic, kMagic, 8) != 0 || !get(f, version) || version != kVersion ||
       !get(f, damped) || !get(f, n_rods) || !get(f, E) || n_rods == 0 || n_rods > (uint64_t(1) << 32))
     return io_error(std::string("erod_load: not an erod checkpoint (version 1): ") + path);
+  // every rod takes at least its header entries and its record: refuse a count
+  // the file cannot hold before sizing anything by it
+  constexpr uint64_t kRodBytes = 8 + 8 + 4 + 8 + 3 * 8 + 4 + 15 * 8 + 4 + 2 * 8;
+  const long here = std::ftell(f);
+  if (here < 0 || std::fseek(f, 0, SEEK_END) != 0) return io_error("erod_load: cannot size the file");
+  const long size = std::ftell(f);
+  if (size < here || std::fseek(f, here, SEEK_SET) != 0) return io_error("erod_load: cannot size the file");
+  if (n_rods > uint64_t(size - here) / kRodBytes) return io_error("erod_load: rod count exceeds the file size");
   std::vector<uint64_t> n_elems(n_rods), n_active(n_rods);
   std::vector<uint32_t> steps_left(n_rods);
   std::vector<double> t(n_rods);

@@ -1,5 +1,7 @@
 """The numpy checkpoint reader against a file packed by hand from the layout
 documented in erod_io.hip (no GPU)."""
+import ctypes
+import os
 import struct
 
 import numpy as np
@@ -53,3 +55,17 @@ def test_reader_rejects_truncation(tmp_path):
     p.write_bytes(data[:-20])
     with pytest.raises(ValueError):
         checkpoint.read(str(p))
+
+
+def test_load_refuses_a_rod_count_the_file_cannot_hold(tmp_path):
+    # a corrupt count must fail on the header, before any allocation or device call
+    data, _ = _pack([4, 4])
+    bad = data[:16] + struct.pack("<Q", 1 << 31) + data[24:]
+    p = tmp_path / "n.ckpt"
+    p.write_bytes(bad)
+    with pytest.raises(ValueError):
+        checkpoint.read_header(str(p))
+    L = ea.load_library()
+    h = ctypes.c_void_p()
+    assert L.erod_load(os.fsencode(str(p)), ctypes.byref(h)) != 0 and not h.value
+    assert b"rod count exceeds" in L.erod_last_error()
