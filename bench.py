@@ -297,7 +297,8 @@ def c3_leg(args, torch, capacity):
            "lambda_per_s": lam, "target_utilization": 0.7, "capacity_element_steps_per_s": capacity,
            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
            "mean_s": float(np.mean(lat)), "max_s": float(np.max(lat)), "throughput": thr, "unit": "element-steps/s",
-           "utilization": thr / capacity, "wall_s": wall, "ticks": ticks, "steps_per_tick": args.chunk,
+           "utilization": thr / capacity, "offered_load_fraction": lam * mean_work / capacity, "wall_s": wall,
+           "ticks": ticks, "steps_per_tick": args.chunk,
            "pool_classes": classes, "pool_bytes": pool_bytes, "hbm_budget_bytes": 200e9}
     log("c3 serving: lambda %.1f/s, p50 %.3f s, p99 %.3f s, %.3e element-steps/s (%.0f%% of capacity), %d ticks"
         % (lam, out["p50_s"], out["p99_s"], thr, 100 * thr / capacity, ticks))
@@ -314,7 +315,7 @@ def main():
     ap.add_argument("--stream-steps", type=int, default=200, help="steps timed at one step per launch")
     ap.add_argument("--c4-steps", type=int, default=2000, help="steps of the contact batch (0: skip it)")
     ap.add_argument("--c4-plain-steps", type=int, default=20)
-    ap.add_argument("--c3-requests", type=int, default=400, help="requests of the serving leg (0: skip it)")
+    ap.add_argument("--c3-requests", type=int, default=2000, help="requests of the serving leg (0: skip it)")
     ap.add_argument("--small-configs", type=int, default=1, help="1: also run the cantilever and the uniform batch")
     ap.add_argument("--tile", default="", help="LANES,EPL: fixed tile shape instead of the autotuner's choice")
     ap.add_argument("--no-cpu", action="store_true")
