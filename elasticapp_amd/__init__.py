@@ -113,6 +113,7 @@ def load_library():
         "erod_server_read": (i32, [vp, u64, u64, i32, P(dbl)]),
         "erod_server_release": (i32, [vp, u64]),
         "erod_server_status": (i32, [vp, P(u64), P(u64), P(u64), P(u64)]),
+        "erod_server_stats": (i32, [vp, P(u64), P(u64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -532,6 +533,12 @@ class Server:
         q, r, e, b = (ctypes.c_uint64() for _ in range(4))
         _check(_LIB.erod_server_status(self._h, ctypes.byref(q), ctypes.byref(r), ctypes.byref(e), ctypes.byref(b)))
         return {"queued": q.value, "running": r.value, "active_elements": e.value, "pool_bytes": b.value}
+
+    def stats(self):
+        """(ticks launched, element-steps advanced) since creation."""
+        t, e = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(_LIB.erod_server_stats(self._h, ctypes.byref(t), ctypes.byref(e)))
+        return t.value, e.value
 
 
 def assemble(block, rod_states):

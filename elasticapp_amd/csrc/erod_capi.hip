@@ -13,11 +13,55 @@ namespace erod {
   static thread_local std::string g_last_error;
   void set_error(const std::string& msg) { g_last_error = msg; }
 
+  namespace {
+    __global__ void k_scatter_rods(const RodInfo* __restrict__ stage, const uint32_t* __restrict__ idx, uint64_t count,
+                                   RodInfo* __restrict__ rods) {
+      const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+      if (i < count) rods[idx[i]] = stage[i];
+    }
+  }  // namespace
+
+  // Upload the rod records the host changed: all of them, or, when only a few
+  // changed (a serving pool admits and releases a handful of slots per tick),
+  // just those, staged and scattered by a small kernel.
   int sync_rods(erod_batch* b) {
-    if (!b->rods_dirty) return EROD_OK;
+    if (!b->rods_dirty) {
+      std::vector<uint32_t>& d = b->dirty_rods;
+      if (d.empty()) return EROD_OK;
+      std::sort(d.begin(), d.end());
+      d.erase(std::unique(d.begin(), d.end()), d.end());
+      if (d.size() * 4 <= b->n_rods) {
+        if (b->rod_stage_capacity < d.size()) {
+          EROD_HIP(hipStreamSynchronize(b->stream));
+          if (b->d_rod_stage) EROD_HIP(hipFree(b->d_rod_stage));
+          if (b->d_rod_stage_idx) EROD_HIP(hipFree(b->d_rod_stage_idx));
+          b->d_rod_stage = nullptr;
+          b->d_rod_stage_idx = nullptr;
+          b->rod_stage_capacity = 0;
+          const uint64_t cap = std::max<uint64_t>(d.size(), 256);
+          EROD_HIP(hipMalloc(&b->d_rod_stage, cap * sizeof(RodInfo)));
+          EROD_HIP(hipMalloc(&b->d_rod_stage_idx, cap * sizeof(uint32_t)));
+          b->rod_stage_capacity = cap;
+        }
+        std::vector<RodInfo> stage(d.size());
+        for (size_t i = 0; i < d.size(); ++i) stage[i] = b->rods[d[i]];
+        EROD_HIP(hipMemcpyAsync(b->d_rod_stage, stage.data(), stage.size() * sizeof(RodInfo), hipMemcpyHostToDevice,
+                                b->stream));
+        EROD_HIP(hipMemcpyAsync(b->d_rod_stage_idx, d.data(), d.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                b->stream));
+        hipLaunchKernelGGL(k_scatter_rods, dim3(static_cast<unsigned>((d.size() + 255) / 256)), dim3(256), 0, b->stream,
+                           b->d_rod_stage, b->d_rod_stage_idx, uint64_t(d.size()), b->d_rods);
+        EROD_HIP(hipGetLastError());
+        // the host staging vectors are released on return
+        EROD_HIP(hipStreamSynchronize(b->stream));
+        d.clear();
+        return EROD_OK;
+      }
+    }
     EROD_HIP(hipMemcpyAsync(b->d_rods, b->rods.data(), sizeof(RodInfo) * b->n_rods, hipMemcpyHostToDevice, b->stream));
     EROD_HIP(hipStreamSynchronize(b->stream));
     b->rods_dirty = false;
+    b->dirty_rods.clear();
     return EROD_OK;
   }
 
@@ -184,6 +228,8 @@ int erod_batch_destroy(erod_batch* b) {
   hipFree(b->d_steps_left);
   hipFree(b->d_copy_pairs);
   hipFree(b->d_rods);
+  hipFree(b->d_rod_stage);
+  hipFree(b->d_rod_stage_idx);
   hipFree(b->d_time);
   hipFree(b->d_flags);
   hipFree(b->d_const32);
@@ -309,7 +355,7 @@ int erod_add_clamp(erod_batch* b, uint64_t rod, uint64_t index, const double x[3
   std::memcpy(ri.clamp_x, x, sizeof(ri.clamp_x));
   std::memcpy(ri.clamp_q, Q, sizeof(ri.clamp_q));
   b->has_clamps = true;
-  b->rods_dirty = true;
+  mark_rod(b, rod);
   b->graph_precision = -1;
   return EROD_OK;
 }
@@ -326,7 +372,7 @@ int erod_add_end_force(erod_batch* b, uint64_t rod, const double F[3], int ramp,
   ri.ramp_start = start;
   ri.ramp_interval = interval;
   b->has_forces = true;
-  b->rods_dirty = true;
+  mark_rod(b, rod);
   b->graph_precision = -1;
   return EROD_OK;
 }
@@ -463,7 +509,7 @@ int erod_set_rod_active_length(erod_batch* b, uint64_t rod, uint64_t n) {
   }
   b->n_active[rod] = n;
   b->rods[rod].n = static_cast<uint32_t>(n);
-  b->rods_dirty = true;
+  mark_rod(b, rod);
   b->damping_dirty = true;
   b->all_full = b->n_active == b->n_elems;
   b->plan.valid = false;
@@ -542,8 +588,8 @@ int erod_copy_rods(erod_batch* dst, const uint64_t* dst_rods, erod_batch* src, c
     dst->steps_left[p.dst_rod] = kNoBudget;
     dst->has_clamps |= (ri.flags & 1u) != 0;
     dst->has_forces |= (ri.flags & 2u) != 0;
+    mark_rod(dst, p.dst_rod);
   }
-  dst->rods_dirty = true;
   dst->budgets_dirty = true;
   dst->all_full = dst->n_active == dst->n_elems;
   dst->plan.valid = false;

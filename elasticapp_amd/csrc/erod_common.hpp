@@ -120,7 +120,11 @@ struct erod_batch {
   std::vector<erod::RodInfo> rods;
   erod::RodInfo* d_rods = nullptr;
   double* d_time = nullptr;  // per-rod clock
-  bool rods_dirty = true;
+  bool rods_dirty = true;                // every rod record must be uploaded
+  std::vector<uint32_t> dirty_rods;      // otherwise: the records changed since the last upload
+  erod::RodInfo* d_rod_stage = nullptr;  // staging of a scattered record upload
+  uint32_t* d_rod_stage_idx = nullptr;
+  uint64_t rod_stage_capacity = 0;
   bool damping_dirty = true;  // per-rod damping rates must be re-read for the fused path
   bool has_clamps = false, has_forces = false;
   erod::Env env{};
@@ -198,5 +202,9 @@ namespace erod {
   int fused_shape_lanes(int shape);
   int fused_shape_epl(int shape);
   int sync_rods(erod_batch* b);
+  // note a changed rod record (uploaded by the next sync_rods)
+  inline void mark_rod(erod_batch* b, uint64_t r) {
+    if (!b->rods_dirty) b->dirty_rods.push_back(static_cast<uint32_t>(r));
+  }
 
 }  // namespace erod

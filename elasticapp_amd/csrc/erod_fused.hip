@@ -106,10 +106,13 @@ namespace erod {
   }
 
   // Default shape without measurement: the smallest capacity that holds the
-  // longest rod; one element per thread up to 512 lanes, four elements per
-  // thread (ILP, one wave per SIMD, no register spills) for 1024.
+  // longest rod; one element per thread up to 256 lanes, two elements per
+  // thread for 512 and 1024 lanes.  (FP32-bulk on ragged batches: 512 lanes at
+  // two elements per thread, two 4-wave workgroups per CU, run 27% faster than
+  // one element per thread in 8-wave workgroups; four elements per thread
+  // spill and run one wave per SIMD.)
   static int default_shape(const erod_batch* b) {
-    static const int pref[] = {0, 1, 2, 3, 7, 5, 4, 6};
+    static const int pref[] = {0, 1, 2, 4, 5, 3, 7, 6};
     for (int si : pref)
       if (shape_feasible(b, si)) return si;
     return -1;

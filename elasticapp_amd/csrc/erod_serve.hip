@@ -123,6 +123,7 @@ struct erod_server {
   std::vector<Request*> running;
   std::deque<uint64_t> finished;  // done, not yet reported
   uint64_t active_elements = 0;
+  uint64_t ticks = 0, element_steps = 0;  // launched ticks and the element-steps they advanced
 };
 
 namespace {
@@ -154,7 +155,7 @@ namespace {
     b->rods[slot].n = 0;
     b->rods[slot].flags = 0;
     b->steps_left[slot] = 0;
-    b->rods_dirty = true;
+    mark_rod(b, slot);
     b->budgets_dirty = true;
     b->plan.valid = false;
     b->graph_precision = -1;
@@ -332,8 +333,14 @@ int erod_server_tick(erod_server* s, uint64_t* done_ids, uint64_t capacity, uint
   }
   // 2. one launch over every slot with budget left
   if (!s->running.empty()) {
+    uint64_t work = 0;
+    for (erod_server::Request* rq : s->running)
+      for (uint64_t slot : rq->slots)
+        work += std::min<uint64_t>(uint64_t(s->steps_per_tick), p->steps_left[slot]) * p->n_active[slot];
     if ((rc = erod_step_fused(p, s->precision, static_cast<uint64_t>(s->steps_per_tick)))) return rc;
     EROD_HIP(hipStreamSynchronize(p->stream));
+    ++s->ticks;
+    s->element_steps += work;
   }
   // 3. completions
   std::vector<erod_server::Request*> still;
@@ -391,6 +398,13 @@ int erod_server_status(const erod_server* s, uint64_t* queued, uint64_t* running
   if (running) *running = s->running.size();
   if (active_elements) *active_elements = s->active_elements;
   if (pool_bytes) *pool_bytes = s->pool_bytes;
+  return EROD_OK;
+}
+
+int erod_server_stats(const erod_server* s, uint64_t* ticks, uint64_t* element_steps) {
+  if (check_server(s)) return EROD_ERR_INVALID;
+  if (ticks) *ticks = s->ticks;
+  if (element_steps) *element_steps = s->element_steps;
   return EROD_OK;
 }
 

@@ -172,6 +172,12 @@ namespace erod {
       check(erod_server_read(s_, id, rod, field, host));
     }
     void release(std::uint64_t id) { check(erod_server_release(s_, id)); }
+    // (ticks launched, element-steps advanced) since creation
+    std::pair<std::uint64_t, std::uint64_t> stats() const {
+      std::uint64_t t = 0, e = 0;
+      check(erod_server_stats(s_, &t, &e));
+      return {t, e};
+    }
 
    private:
     erod_server* s_ = nullptr;

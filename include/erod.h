@@ -223,6 +223,10 @@ int erod_server_read(erod_server* s, uint64_t id, uint64_t rod, int field, doubl
 int erod_server_release(erod_server* s, uint64_t id);
 int erod_server_status(const erod_server* s, uint64_t* queued, uint64_t* running, uint64_t* active_elements,
                        uint64_t* pool_bytes);
+/* Totals since creation: ticks that launched, and element-steps they advanced
+ * (each rod counts min(steps per tick, its budget left) steps times its active
+ * length), so that throughput over a window is a difference of two readings. */
+int erod_server_stats(const erod_server* s, uint64_t* ticks, uint64_t* element_steps);
 
 /* ---- checkpoints ------------------------------------------------------------ */
 

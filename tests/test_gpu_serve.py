@@ -117,6 +117,10 @@ def test_server_mix_equals_alone(precision):
             done.add(rid)
         ticks += 1
         assert ticks < 1000
+    # every rod advanced exactly its request's steps
+    n_ticks, element_steps = srv.stats()
+    assert 0 < n_ticks <= ticks
+    assert element_steps == sum(sum(wl.n_elems[r] for r in rods) * steps for _, rods, steps in reqs)
     # each request: equal to the same request served alone, and to its rods stepped directly
     for rid, rods, steps in (reqs[0], reqs[7], reqs[15]):
         alone = Server(lib, classes, precision=precision, steps_per_tick=50, hbm_budget_bytes=2e9)
