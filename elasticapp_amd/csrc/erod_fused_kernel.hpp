@@ -364,8 +364,16 @@ namespace erod {
         out.lag[c] = lag[c];
         out.unst[c] = jw[c] * rate;
       }
-      // Voronoi k
-      if (has_vor) {
+      // Voronoi k.  A rod's last element has none.  FP32-bulk computes the
+      // terms anyway: its constants hold B = 0, kappa0 = 0 and D0 = 1, its
+      // neighbour inputs are finite values of the next lane and its log map
+      // clamps its argument, so the couple M = B (kappa - kappa0) and both
+      // terms below are exactly zero, with no branch (which every wave with a
+      // rod end in it would take both sides of).  The FP64 path keeps the
+      // reference's acos(tr / 2 - 1/2 - 1e-10), which is not defined for a
+      // relative rotation of pi (a rod's last director against an idle lane's
+      // identity), so it branches.
+      if (sizeof(R) == 4 || has_vor) {
         R l1;
         if constexpr (sizeof(R) == 8) {
           const double f0 = xk2[0] - xk1[0], f1 = xk2[1] - xk1[1], f2 = xk2[2] - xk1[2];
@@ -649,7 +657,11 @@ namespace erod {
             }
           }
           const bool has_vor = !last;
-          if (has_vor) {
+          // the next lane's director and node difference; for a rod's last
+          // element they belong to the next rod (or to an idle lane: identity
+          // director, zero state) and only meet its zero Voronoi constants
+          // (FP32-bulk) or are not used (FP64)
+          {
             if (j + 1 < P) {
 #pragma unroll
               for (int m = 0; m < 9; ++m) qk1[m] = q[j + 1 < P ? j + 1 : j][m];
@@ -687,9 +699,6 @@ namespace erod {
 #pragma unroll
               for (int c = 0; c < 3; ++c) dn[c] = static_cast<float>(xk2[c] - xk1[c]);
             }
-          } else {
-#pragma unroll
-            for (int m = 0; m < 9; ++m) qk1[m] = 0.0;
           }
           ElemConst<R> K;
           if constexpr (kCache) {
