@@ -456,8 +456,9 @@ namespace erod {
       constexpr int kS1 = kCache ? 15 : 18;
       __shared__ double s1[kS1][NT];
       __shared__ float s5[kCache ? 3 : 1][NT];
-      // exchange 2: per thread, its last element's c, P, R
-      __shared__ R s3[9][NT];
+      // exchange 2: per thread, its last element's c, P, R, in column tid + 1
+      // (column 0 stays zero: the -1 terms of the tile's first element)
+      __shared__ R s3[9][NT + 1];
       // FP32-bulk: 18 constants of every lane of the tile, for the whole launch:
       //   0 m, 1 l0, 2-4 sig0, 5-7 S, 8-10 J, 11 D0, 12-14 kap0, 15-17 B
       // (1/m, 1/J, 1/l0, 1/D0 are formed when used; the defaults of an element
@@ -577,6 +578,7 @@ namespace erod {
         __syncthreads();
       }
 
+      if (tid < 9) s3[tid][0] = R(0);  // read only after the barriers of the first step
       for (int step = 0; step < nsteps; ++step) {
         // a rod whose budget is spent keeps its state (its element forces are
         // still evaluated, since the tile moves in lockstep, but never applied)
@@ -726,34 +728,27 @@ namespace erod {
         // ---- exchange 2 -----------------------------------------------------------
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          s3[c][tid] = mid[P - 1].c[c];
-          s3[3 + c][tid] = mid[P - 1].P[c];
-          s3[6 + c][tid] = mid[P - 1].Rv[c];
+          s3[c][tid + 1] = mid[P - 1].c[c];
+          s3[3 + c][tid + 1] = mid[P - 1].P[c];
+          s3[6 + c][tid + 1] = mid[P - 1].Rv[c];
         }
         __syncthreads();
-        const int tp = tid > 0 ? tid - 1 : 0;
 #pragma unroll
         for (int j = 0; j < P; ++j) {
           if (!live[j]) continue;
           const int k = kk[j];
           const bool first = k == 0, last = k == nn[j] - 1;
-          R cm[3] = {R(0), R(0), R(0)}, Pm[3] = {R(0), R(0), R(0)}, Rm[3] = {R(0), R(0), R(0)};
-          if (!first) {
-            if (j > 0) {
+          // the -1 element's terms (the previous thread's from exchange 2 for
+          // j = 0).  Before a rod's first element sits the previous rod's last
+          // element, whose P and R are zero (no Voronoi region), or the zero
+          // column; only its c is masked.
+          R cm[3], Pm[3], Rm[3];
 #pragma unroll
-              for (int c = 0; c < 3; ++c) {
-                cm[c] = mid[j > 0 ? j - 1 : 0].c[c];
-                Pm[c] = mid[j > 0 ? j - 1 : 0].P[c];
-                Rm[c] = mid[j > 0 ? j - 1 : 0].Rv[c];
-              }
-            } else {
-#pragma unroll
-              for (int c = 0; c < 3; ++c) {
-                cm[c] = s3[c][tp];
-                Pm[c] = s3[3 + c][tp];
-                Rm[c] = s3[6 + c][tp];
-              }
-            }
+          for (int c = 0; c < 3; ++c) {
+            cm[c] = j > 0 ? mid[j > 0 ? j - 1 : 0].c[c] : s3[c][tid];
+            Pm[c] = j > 0 ? mid[j > 0 ? j - 1 : 0].P[c] : s3[3 + c][tid];
+            Rm[c] = j > 0 ? mid[j > 0 ? j - 1 : 0].Rv[c] : s3[6 + c][tid];
+            cm[c] = first ? R(0) : cm[c];
           }
           const uint64_t s = slot[j];
           const Mid<R>& md = mid[j];
