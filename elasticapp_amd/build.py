@@ -2,10 +2,12 @@
 
     python -m elasticapp_amd.build          # or: python elasticapp_amd/build.py
 
-The op-by-op FP64 kernels are compiled with -ffp-contract=off (every product
-and sum rounded on its own, like the CPU restatement they are checked
-against); the fused kernels may contract into FMAs.  Packed FP32 instructions
-are disabled for all device code.
+The op-by-op FP64 kernels and both fused instantiations are compiled with
+-ffp-contract=off: the op-by-op and fused FP64 kernels round every product and
+sum on their own, like the CPU restatement they are checked against, and the
+FP32-bulk kernel's only fused multiply-adds are the explicit ones in its
+source, so its results do not depend on the tile shape.  Packed FP32
+instructions are disabled for all device code.
 """
 import os
 import shutil
