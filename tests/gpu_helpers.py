@@ -9,9 +9,12 @@ from oracle import pyoracle
 STATE = ("Position", "Velocity", "Director", "AngularVelocity")
 
 
-def random_workload(n_rods, n_min, n_max, seed, steps=100, forces=True, kappa=0.5, vstd=1e-2):
+def random_workload(n_rods, n_min, n_max, seed, steps=100, forces=True, kappa=0.5, vstd=1e-2, lengths=None):
+    """Randomly bent, moving, loaded rods; `lengths` fixes the element counts
+    (n_rods, n_min and n_max are then unused)."""
     rng = np.random.default_rng(seed)
-    n_elems = rng.integers(n_min, n_max + 1, n_rods)
+    n_elems = rng.integers(n_min, n_max + 1, n_rods) if lengths is None else np.array(lengths)
+    n_rods = len(n_elems)
     states, fl = [], []
     dts = []
     for i in range(n_rods):

@@ -17,7 +17,8 @@ Tolerances
 import numpy as np
 import pytest
 
-from elasticapp_amd import FP32_BULK, FP64, DeviceBlock, straight_rod_state
+from elasticapp_amd import FP32_BULK, FP64, DeviceBlock, ErodError, straight_rod_state
+from elasticapp_amd.workloads import Workload
 from tests.golden_fixtures import as_field, load, rel_err_ok
 from tests.gpu_helpers import STATE, device_block, oracle_system, random_workload, rel_state_error
 
@@ -136,6 +137,56 @@ def test_fused_is_independent_of_tiling(lanes, epl, precision):
     for r in range(len(wl.n_elems)):
         for t in STATE:
             assert np.array_equal(blk.get_rod(t, r), ref.get_rod(t, r)), (lanes, epl, r, t)
+
+
+def _edge_lengths():
+    # rods at the tile capacities (1024 and 512 lanes) and one off them, and
+    # hundreds of two- and three-element rods that the planner packs into the
+    # room the long rods leave
+    return [1024, 1023, 513, 512, 511, 257, 256, 255, 2, 3] + [2] * 300 + [3] * 60 + [1024]
+
+
+def test_edge_lengths_fused_fp64_equals_op_by_op():
+    wl = random_workload(0, 0, 0, seed=23, steps=100, lengths=_edge_lengths())
+    plain = device_block(wl)
+    plain.step(wl.dt, wl.steps)
+    fused = device_block(wl)
+    fused.set_fused_config(0, 0, 50)
+    fused.step_fused(wl.steps, FP64)
+    assert fused.fused_config()[0] == 1024
+    for r in range(len(wl.n_elems)):
+        for t in STATE:
+            a, b = fused.get_rod(t, r), plain.get_rod(t, r)
+            assert np.abs(a - b).max() <= 1e-12 * max(np.abs(b).max(), 1.0), (r, t)
+    f32 = device_block(wl)
+    f32.set_fused_config(0, 0, 50)
+    f32.step_fused(wl.steps, FP32_BULK)
+    for r in range(len(wl.n_elems)):
+        x, y = f32.get_rod("Position", r), plain.get_rod("Position", r)
+        L = np.linalg.norm(np.diff(y, axis=1), axis=0).sum()
+        assert np.abs(x - y).max() <= 1e-6 * L, r
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32_BULK])
+def test_edge_lengths_are_independent_of_placement(precision):
+    """The same rods in reverse order land in other tiles and lanes (the
+    planner keeps the order of equal lengths) and step bitwise the same; a
+    shape with more elements per thread than the shortest rod is refused."""
+    lengths = _edge_lengths()
+    wl = random_workload(0, 0, 0, seed=29, steps=60, lengths=lengths)
+    rev = Workload("reversed", wl.n_elems[::-1], wl.states[::-1], wl.dt, wl.steps,
+                   forces=[(len(lengths) - 1 - f[0],) + tuple(f[1:]) for f in wl.forces])
+    a, b = device_block(wl), device_block(rev)
+    for blk in (a, b):
+        blk.set_fused_config(1024, 2, 30)
+        blk.step_fused(wl.steps, precision)
+    n = len(lengths)
+    for r in range(n):
+        for t in STATE:
+            assert np.array_equal(a.get_rod(t, r), b.get_rod(t, n - 1 - r)), (r, t)
+    a.set_fused_config(1024, 4, 30)
+    with pytest.raises(ErodError):
+        a.step_fused(1, precision)
 
 
 def test_autotune_keeps_state_and_picks_a_shape():
