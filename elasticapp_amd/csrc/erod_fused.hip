@@ -202,6 +202,13 @@ namespace erod {
       lane[t] += static_cast<uint32_t>(b->n_active[order[i]]);
     }
     for (size_t t = 0; t < n_tiles; ++t) tiles[t].lanes = lane[t];
+    // the kernel stages a tile's rod table in NT entries of LDS (rods have at
+    // least two elements, and at least P: a feasible shape never exceeds it)
+    for (size_t t = 0; t < n_tiles; ++t)
+      if (tiles[t].count > uint32_t(fused::kShapes[si].NT)) {
+        set_error("plan_tiles: more rods in a tile than threads");
+        return EROD_ERR_INVALID;
+      }
     // the plan arrays grow to the largest plan seen (a tile or a rod entry per
     // rod at most) and are reused when the rod set changes
     if (b->plan.rod_capacity < b->n_rods) {

@@ -481,22 +481,37 @@ namespace erod {
       R tail_m = R(0), tail_im = R(0);
       double xt[3] = {0, 0, 0}, vt[3] = {0, 0, 0};
       int tailj = -1;
+      // the tile's rod table (first lane, rod) in LDS, in the exchange-1
+      // buffer until the first step: a tile holds at most NT rods (each rod
+      // has at least two elements, and at least P), so one pass loads it and
+      // the lookups below are LDS searches, not chains of dependent global loads
+      uint32_t* const t_lane0 = reinterpret_cast<uint32_t*>(&s1[0][0]);
+      uint32_t* const t_rod = t_lane0 + NT;
+      if (uint32_t(tid) < tile.count) {
+        t_lane0[tid] = tile_lane0[tile.first + tid];
+        t_rod[tid] = tile_rods[tile.first + tid];
+      }
+      __syncthreads();
+      // the table entry of lane el: the last with first lane <= el
+      auto find = [&](uint32_t el) {
+        uint32_t lo = 0, hi = tile.count;
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (t_lane0[mid] <= el) lo = mid;
+          else hi = mid;
+        }
+        return lo;
+      };
       // pass 1: which rod and slot each of the thread's elements is
 #pragma unroll
       for (int j = 0; j < P; ++j) {
         const uint32_t el = uint32_t(tid) * P + j;
         valid[j] = el < tile.lanes;
-        // rod of this element: last entry with lane0 <= el
-        uint32_t lo = 0, hi = tile.count;
-        while (hi - lo > 1) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (tile_lane0[tile.first + mid] <= el) lo = mid;
-          else hi = mid;
-        }
-        const uint32_t r = tile_rods[tile.first + lo];
+        const uint32_t e = find(el);
+        const uint32_t r = t_rod[e];
         rodi[j] = int(r);
         const RodInfo& ri = rods[r];
-        kk[j] = valid[j] ? int(el - tile_lane0[tile.first + lo]) : 0;
+        kk[j] = valid[j] ? int(el - t_lane0[e]) : 0;
         nn[j] = int(ri.n);
         slot[j] = ri.start + uint64_t(kk[j]);
         dt[j] = ri.dt;
@@ -549,14 +564,9 @@ namespace erod {
         for (int i = 0; i < P; ++i) {
           const uint32_t el = uint32_t(tid) + uint32_t(i) * NT;
           if (el >= tile.lanes) continue;
-          uint32_t lo = 0, hi = tile.count;
-          while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (tile_lane0[tile.first + mid] <= el) lo = mid;
-            else hi = mid;
-          }
-          const RodInfo& ri = rods[tile_rods[tile.first + lo]];
-          const uint32_t k = el - tile_lane0[tile.first + lo];
+          const uint32_t e = find(el);
+          const RodInfo& ri = rods[t_rod[e]];
+          const uint32_t k = el - t_lane0[e];
           const uint64_t sl = ri.start + k;
           const uint32_t ce = cidx(el);
           sc[0][ce] = C.m[sl];
@@ -578,6 +588,7 @@ namespace erod {
         __syncthreads();
       }
 
+      if constexpr (!kCache) __syncthreads();  // every lookup in the rod table done before s1 is written
       if (tid < 9) s3[tid][0] = R(0);  // read only after the barriers of the first step
       for (int step = 0; step < nsteps; ++step) {
         // a rod whose budget is spent keeps its state (its element forces are
