@@ -466,7 +466,6 @@ namespace erod {
       // at j NT + t, so that a wave reading element j of its threads reads
       // consecutive words.
       __shared__ float sc[kCache ? 18 : 1][kCache ? P * NT : 1];
-      auto cidx = [](uint32_t el) { return (el % P) * NT + el / P; };
 
       // ---- per element: rod, local index, slots, state -----------------------
       int rodi[P];
@@ -557,18 +556,15 @@ namespace erod {
           tail_im = C.invm[slot[j] + 1];
         }
       }
+      // FP32-bulk: the thread's constants into the LDS table, from the slots
+      // found above
       if constexpr (kCache) {
-        // lane el = tid + i NT: consecutive lanes of a rod are consecutive slots
         const uint64_t V = E - 1;
 #pragma unroll
-        for (int i = 0; i < P; ++i) {
-          const uint32_t el = uint32_t(tid) + uint32_t(i) * NT;
-          if (el >= tile.lanes) continue;
-          const uint32_t e = find(el);
-          const RodInfo& ri = rods[t_rod[e]];
-          const uint32_t k = el - t_lane0[e];
-          const uint64_t sl = ri.start + k;
-          const uint32_t ce = cidx(el);
+        for (int j = 0; j < P; ++j) {
+          if (!valid[j]) continue;
+          const uint64_t sl = slot[j];
+          const uint32_t ce = uint32_t(j) * NT + tid;  // lane tid P + j
           sc[0][ce] = C.m[sl];
           sc[1][ce] = C.l0[sl];
 #pragma unroll
@@ -577,7 +573,7 @@ namespace erod {
             sc[5 + c][ce] = C.S[c * E + sl];
             sc[8 + c][ce] = C.J[c * E + sl];
           }
-          const bool vor = k + 1 < ri.n;
+          const bool vor = kk[j] + 1 < nn[j];
           sc[11][ce] = vor ? C.D0[sl] : 1.0f;
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
@@ -585,9 +581,8 @@ namespace erod {
             sc[15 + c][ce] = vor ? C.B[c * V + sl] : 0.0f;
           }
         }
-        __syncthreads();
+        __syncthreads();  // the constant table is written
       }
-
       if constexpr (!kCache) __syncthreads();  // every lookup in the rod table done before s1 is written
       if (tid < 9) s3[tid][0] = R(0);  // read only after the barriers of the first step
       for (int step = 0; step < nsteps; ++step) {
@@ -715,7 +710,7 @@ namespace erod {
           }
           ElemConst<R> K;
           if constexpr (kCache) {
-            const uint32_t el = uint32_t(j) * NT + tid;  // cidx(tid P + j)
+            const uint32_t el = uint32_t(j) * NT + tid;  // lane tid P + j
             K.l0 = sc[1][el];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -765,8 +760,8 @@ namespace erod {
           const Mid<R>& md = mid[j];
           R mk, mkn, im, iJ[3];
           if constexpr (kCache) {
-            const uint32_t el = uint32_t(j) * NT + tid;                   // cidx(tid P + j)
-            const uint32_t eln = j + 1 < P ? el + NT : uint32_t(tid) + 1;  // cidx(tid P + j + 1)
+            const uint32_t el = uint32_t(j) * NT + tid;                   // lane tid P + j
+            const uint32_t eln = j + 1 < P ? el + NT : uint32_t(tid) + 1;  // lane tid P + j + 1
             mk = sc[0][el];
             mkn = last ? tail_m : sc[0][kCache ? eln : 0];
             im = rcp_(mk);
