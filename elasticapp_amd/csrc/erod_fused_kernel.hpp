@@ -480,6 +480,13 @@ namespace erod {
       R tail_m = R(0), tail_im = R(0);
       double xt[3] = {0, 0, 0}, vt[3] = {0, 0, 0};
       int tailj = -1;
+      // FP32-bulk without environment: the end force on the rod's last node
+      // is kept in registers for the launch (the ramp's times are read only
+      // while a ramp is set), instead of two dependent rod-record loads per
+      // step; the other instantiations have no registers to spare for it
+      constexpr bool kTailForce = kCache && !kEnv;
+      double tforce[3] = {0, 0, 0};
+      int tramp = EROD_RAMP_NONE;
       // the tile's rod table (first lane, rod) in LDS, in the exchange-1
       // buffer until the first step: a tile holds at most NT rods (each rod
       // has at least two elements, and at least P), so one pass loads it and
@@ -554,6 +561,12 @@ namespace erod {
           }
           tail_m = C.m[slot[j] + 1];
           tail_im = C.invm[slot[j] + 1];
+          if (kTailForce && (rflags[j] & 2u)) {
+            const RodInfo& ri = rods[rodi[j]];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) tforce[c] = ri.force[c];
+            tramp = ri.ramp;
+          }
         }
       }
       // FP32-bulk: the thread's constants into the LDS table, from the slots
@@ -814,10 +827,20 @@ namespace erod {
             const R mgn = tail_m * gf;
             double fen[3] = {0.0, 0.0, 0.0};
             if (rflags[j] & 2u) {
-              const RodInfo& ri = rods[rodi[j]];
-              const double rv = ramp_value(ri.ramp, tm[j], ri.ramp_start, ri.ramp_interval);
+              if constexpr (kTailForce) {
+                double rv = 1.0;
+                if (tramp != EROD_RAMP_NONE) {
+                  const RodInfo& ri = rods[rodi[j]];
+                  rv = ramp_value(tramp, tm[j], ri.ramp_start, ri.ramp_interval);
+                }
 #pragma unroll
-              for (int c = 0; c < 3; ++c) fen[c] = ri.force[c] * rv;
+                for (int c = 0; c < 3; ++c) fen[c] = tforce[c] * rv;
+              } else {
+                const RodInfo& ri = rods[rodi[j]];
+                const double rv = ramp_value(ri.ramp, tm[j], ri.ramp_start, ri.ramp_interval);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) fen[c] = ri.force[c] * rv;
+              }
             }
             if constexpr (kEnv) node_environment(env, static_cast<double>(tail_m), xt, vt, fen);
 #pragma unroll
