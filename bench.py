@@ -269,7 +269,8 @@ def c3_leg(args, torch, capacity):
     mean_work = float(np.mean([n_lib[r].sum() * h for r, h in serving_requests(20000, len(n_lib), seed=99)]))
     classes = [(64, 8192), (128, 8192), (256, 8192), (512, 8192)]
     # the server's capacity: a backlog keeps the queue non-empty over the timed ticks
-    sat = Server(lib, classes, precision=FP32_BULK, steps_per_tick=args.chunk, hbm_budget_bytes=200e9)
+    spt = args.c3_steps_per_tick
+    sat = Server(lib, classes, precision=FP32_BULK, steps_per_tick=spt, hbm_budget_bytes=200e9)
     for i, (rods, h) in enumerate(serving_requests(3000, len(n_lib), seed=3006)):
         sat.submit(i, rods, h)
     for _ in range(20):
@@ -288,7 +289,7 @@ def c3_leg(args, torch, capacity):
     lam = 0.7 * server_capacity / mean_work
     rng = np.random.default_rng(3005)
     arrivals = np.cumsum(rng.exponential(1.0 / lam, len(reqs)))
-    srv = Server(lib, classes, precision=FP32_BULK, steps_per_tick=args.chunk, hbm_budget_bytes=200e9)
+    srv = Server(lib, classes, precision=FP32_BULK, steps_per_tick=spt, hbm_budget_bytes=200e9)
     pool_bytes = srv.status()["pool_bytes"]
     lat = np.full(len(reqs), np.nan)
     nxt = done = ticks = 0
@@ -320,7 +321,7 @@ def c3_leg(args, torch, capacity):
            "mean_s": float(np.mean(lat)), "max_s": float(np.max(lat)), "throughput": thr, "unit": "element-steps/s",
            "utilization": thr / server_capacity, "offered_load_fraction": lam * mean_work / server_capacity,
            "wall_s": wall,
-           "ticks": ticks, "steps_per_tick": args.chunk,
+           "ticks": ticks, "steps_per_tick": spt,
            "pool_classes": classes, "pool_bytes": pool_bytes, "hbm_budget_bytes": 200e9}
     log("c3 serving: lambda %.1f/s, p50 %.3f s, p99 %.3f s, %.3e element-steps/s (%.0f%% of the server's capacity), "
         "%d ticks" % (lam, out["p50_s"], out["p99_s"], thr, 100 * thr / server_capacity, ticks))
@@ -338,6 +339,7 @@ def main():
     ap.add_argument("--c4-steps", type=int, default=2000, help="steps of the contact batch (0: skip it)")
     ap.add_argument("--c4-plain-steps", type=int, default=20)
     ap.add_argument("--c3-requests", type=int, default=2000, help="requests of the serving leg (0: skip it)")
+    ap.add_argument("--c3-steps-per-tick", type=int, default=500, help="fused steps per server tick")
     ap.add_argument("--small-configs", type=int, default=1, help="1: also run the cantilever and the uniform batch")
     ap.add_argument("--tile", default="", help="LANES,EPL: fixed tile shape instead of the autotuner's choice")
     ap.add_argument("--no-cpu", action="store_true")
