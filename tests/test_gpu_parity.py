@@ -17,7 +17,7 @@ Tolerances
 import numpy as np
 import pytest
 
-from elasticapp_amd import FP32_BULK, FP64, DeviceBlock, ErodError, straight_rod_state
+from elasticapp_amd import FIELDS, FP32_BULK, FP64, DeviceBlock, ErodError, straight_rod_state
 from elasticapp_amd.workloads import Workload
 from tests.golden_fixtures import as_field, load, rel_err_ok
 from tests.gpu_helpers import STATE, device_block, oracle_system, random_workload, rel_state_error
@@ -289,6 +289,53 @@ def test_hooks_op_by_op_and_fused_match_oracle():
     # clamped ends stay put
     for r in range(3):
         assert np.array_equal(b.get_rod("Position", r)[:, 0], wl.states[r]["Position"][:, 0])
+
+
+def _verlet_by_calls(blk, dt, n, t=0.0):
+    # the reference stepper's sequence over one block (PositionVerlet.hpp:49-88,
+    # SymplecticSteppers.hpp:152-216): kinematic half step, post-kinematic
+    # hooks, internal dynamics (external loads zeroed first), pre-dynamic hooks,
+    # dynamic step, post-dynamic hooks, kinematic half step
+    for _ in range(n):
+        blk.kinematic_update(dt)
+        t += 0.5 * dt
+        blk.apply_post_kinematic(t)
+        blk.compute_internal_dynamics(t)
+        blk.apply_pre_dynamic(t)
+        blk.dynamic_update(dt)
+        blk.apply_post_dynamic(t)
+        t += 0.5 * dt
+        blk.kinematic_update(dt)
+    return t
+
+
+def test_per_block_calls_compose_the_step():
+    """The three per-block calls and the hook stages, driven one by one from
+    the host as the reference's stepper drives a block, equal erod_step
+    bitwise in every field (hooks: clamps, ramped end forces, gravity, plane
+    contact with friction)."""
+    wl = _hooked_workload()
+    a, b = device_block(wl), device_block(wl)
+    _verlet_by_calls(a, wl.dt, 60)
+    b.step(wl.dt, 60)
+    for tag in FIELDS:
+        for r in range(len(wl.n_elems)):
+            assert np.array_equal(a.get_rod(tag, r), b.get_rod(tag, r)), (tag, r)
+
+
+@pytest.mark.parametrize("case,section", [p for p in _golden_cases() if p.values[1]["steps"] > 0])
+def test_golden_per_block_calls(case, section):
+    """The golden steps through the three per-block calls, every variable the
+    reference's test compares, at its tolerances."""
+    blk = _golden_block(case)
+    _verlet_by_calls(blk, case["dt"], section["steps"])
+    bad = []
+    for rod, vars_ in section["expected"].items():
+        for tag, entry in vars_.items():
+            ok = rel_err_ok(blk.get_rod(tag, int(rod)), as_field(entry, tag), section["epsilon"])
+            if not ok.all():
+                bad.append("rod%s %s %d/%d" % (rod, tag, (~ok).sum(), ok.size))
+    assert not bad, bad
 
 
 def test_finite_flags():
