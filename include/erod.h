@@ -146,7 +146,9 @@ int erod_set_rod_time(erod_batch* b, const double* time_per_rod);
  * totalling at most lanes_per_tile elements (64 ... 1024), each thread holding
  * elements_per_thread of them (1, 2 or 4); 0 leaves a value free.
  * steps_per_launch: steps a workgroup advances with its rods' state held on
- * chip (0 = 100).  The rods of a batch share one shape. */
+ * chip (0 = 100).  The rods of a batch share one shape.  Without a request or
+ * an autotune, a batch takes the smallest capacity that holds its longest rod
+ * (one element per thread up to 256 lanes, two from 512). */
 int erod_set_fused_config(erod_batch* b, int lanes_per_tile, int elements_per_thread, int steps_per_launch);
 int erod_get_fused_config(const erod_batch* b, int* lanes_per_tile, int* elements_per_thread, int* steps_per_launch);
 /* Times every feasible tile shape on `trial_steps` steps from a snapshot of
