@@ -292,7 +292,7 @@ def c3_leg(args, torch, capacity):
     srv = Server(lib, classes, precision=FP32_BULK, steps_per_tick=spt, hbm_budget_bytes=200e9)
     pool_bytes = srv.status()["pool_bytes"]
     lat = np.full(len(reqs), np.nan)
-    nxt = done = ticks = 0
+    nxt = done = ticks = non_finite = 0
     t0 = time.perf_counter()
     while done < len(reqs):
         now = time.perf_counter() - t0
@@ -304,7 +304,8 @@ def c3_leg(args, torch, capacity):
             time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
             continue
         for rid in srv.tick():
-            srv.read(rid, 0, "Position")         # the result leaves the device
+            x = srv.read(rid, 0, "Position")     # the result leaves the device
+            non_finite += int(not np.isfinite(x).all())
             srv.release(rid)
             lat[rid] = time.perf_counter() - t0 - arrivals[rid]
             done += 1
@@ -319,6 +320,7 @@ def c3_leg(args, torch, capacity):
            "server_capacity_ticks": k1 - k0, "capacity_element_steps_per_s": capacity,
            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
            "mean_s": float(np.mean(lat)), "max_s": float(np.max(lat)), "throughput": thr, "unit": "element-steps/s",
+           "non_finite_results": non_finite,
            "utilization": thr / server_capacity, "offered_load_fraction": lam * mean_work / server_capacity,
            "wall_s": wall,
            "ticks": ticks, "steps_per_tick": spt,

@@ -21,6 +21,14 @@ wave speed sqrt(E / rho) = sqrt(1e6 / 5e3); for other materials the same
 Courant number is kept: dt_i = 0.01 (L_i / n_i) sqrt(1e6 / 5e3) / sqrt(E_i / rho_i),
 and a batch uses the smallest dt of its rods (the reference steps every
 system with one dt).
+
+A rod stepped with its own dt (the serving library) must also respect two
+limits of the explicit step that the axial rule does not see when the radius
+r is small against L / n, taken at a tenth of their critical values:
+  shear-rotation mode   omega = (2 / r) sqrt(G alpha / rho), alpha = 27/28:
+                        dt < 2 / omega = r sqrt(rho / (G alpha))
+  torque damping        d(w)/dt = -gamma (m / J) w with m / J = 4 / r^2:
+                        dt < 2 / (4 gamma / r^2) = r^2 / (2 gamma)
 """
 from dataclasses import dataclass, field
 
@@ -49,8 +57,16 @@ class Workload:
         return int(sum(self.n_elems))
 
 
-def stable_dt(length, n, youngs, density):
-    return 0.01 * (length / n) * C_REF / np.sqrt(youngs / density)
+def stable_dt(length, n, youngs, density, radius=None, shear=None, damping=None):
+    """The axial Courant rule; with radius, shear modulus and torque damping
+    rate also a tenth of the shear-rotation and damping limits."""
+    dt = 0.01 * (np.asarray(length) / n) * C_REF / np.sqrt(np.asarray(youngs) / density)
+    if radius is not None:
+        r = np.asarray(radius)
+        rot = r * np.sqrt(np.asarray(density) / (np.asarray(shear) * (27.0 / 28.0)))
+        damp = r * r / (2.0 * np.maximum(np.asarray(damping), 1e-300))
+        dt = np.minimum(dt, 0.1 * np.minimum(rot, damp))
+    return dt
 
 
 def timoshenko_c0(steps=50000):
@@ -101,8 +117,9 @@ def ragged_c2(n_rods=4096, steps=10000, seed=2002, n_min=16, n_max=1024):
         states.append(_randomize(st, rng))
         forces.append((i, F[i].copy(), 0, 0.0, 1.0))
     dts = stable_dt(L, n_elems, E, rho)
+    rod_dt = stable_dt(L, n_elems, E, rho, r, E / (1.0 + 0.99), gam)
     return Workload("ragged_c2", [int(x) for x in n_elems], states, float(dts.min()), steps, forces=forces,
-                    info=dict(rod_dt_min=float(dts.min()), rod_dt_max=float(dts.max())))
+                    info=dict(rod_dt_min=float(dts.min()), rod_dt_max=float(dts.max()), rod_dt=rod_dt))
 
 
 class UniformRods:
@@ -191,7 +208,8 @@ def contact_c4(n_rods=100000, n=128, steps=2000, seed=4004, k=1e3, c=10.0, mu=0.
 def serving_library(n_templates=2048, seed=3003, n_min=32, n_max=512):
     """Template rods for the serving workload: n log-uniform in [32, 512],
     materials and perturbations as in the ragged batch, a constant tip force
-    N(0, 1) N, and each rod's own stable dt (returned per rod)."""
+    N(0, 1) N, and each rod's own stable dt (returned per rod; the axial rule
+    and the rotational limits)."""
     rng = np.random.default_rng(seed)
     n_elems = np.clip(np.rint(np.exp(rng.uniform(np.log(n_min), np.log(n_max), n_templates))).astype(int), n_min, n_max)
     L = rng.uniform(0.5, 3.0, n_templates)
@@ -206,7 +224,7 @@ def serving_library(n_templates=2048, seed=3003, n_min=32, n_max=512):
                                 rho[i], E[i], E[i] / (1.0 + 0.99), gam[i], gam[i])
         states.append(_randomize(st, rng))
         forces.append((i, F[i].copy(), 0, 0.0, 1.0))
-    dts = stable_dt(L, n_elems, E, rho)
+    dts = stable_dt(L, n_elems, E, rho, r, E / (1.0 + 0.99), gam)
     wl = Workload("serving_library", [int(x) for x in n_elems], states, float(dts.min()), 0, forces=forces)
     return wl, dts
 

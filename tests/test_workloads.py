@@ -65,3 +65,20 @@ def test_ragged_c2_is_seeded():
     assert a.n_elems == b.n_elems and a.dt == b.dt
     assert np.array_equal(a.states[10]["Velocity"], b.states[10]["Velocity"])
     assert min(a.n_elems) >= 16 and max(a.n_elems) <= 1024
+
+
+def test_per_rod_dt_respects_the_rotational_limits():
+    """The per-rod rule keeps dt * omega below 0.2 for the shear-rotation mode,
+    omega = (2 / r) sqrt(G alpha / rho), and dt * gamma * 4 / r^2 below 0.2
+    for the torque damping; for the cantilever's rod (r > L / n) it is the
+    axial rule alone."""
+    from elasticapp_amd.workloads import serving_library, stable_dt
+    L, n, E, rho, r, gam = 1.9, 17, 1.26e5, 2256.0, 0.0077, 0.3
+    G = E / 1.99
+    dt = float(stable_dt(L, n, E, rho, r, G, gam))
+    assert dt < float(stable_dt(L, n, E, rho))
+    omega = 2.0 / r * np.sqrt(G * 27.0 / 28.0 / rho)
+    assert dt * omega <= 0.2 * (1 + 1e-12) and dt * gam * 4.0 / r ** 2 <= 0.2 * (1 + 1e-12)
+    assert float(stable_dt(3.0, 100, 1e6, 5e3, 0.25, 1e6 / 1.99, 0.1)) == float(stable_dt(3.0, 100, 1e6, 5e3))
+    wl, dts = serving_library(n_templates=64)
+    assert dts.shape == (64,) and (dts > 0).all() and float(dts.min()) == wl.dt
