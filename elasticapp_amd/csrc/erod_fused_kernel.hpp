@@ -33,7 +33,10 @@
 //   cancellation-prone parts (node differences and the relative rotation of
 //   neighbouring directors) formed in FP64 before rounding.  The log map is
 //   the reference's acos(cos(theta) - 1e-10) rewritten without cancellation
-//   as theta = 2 asin(sqrt(sin^2(theta/2) + 5e-11)); element lengths come
+//   as theta = 2 asin(sqrt(sin^2(theta/2) + 5e-11)) (a series for small
+//   angles; past a quarter turn from the cosine side, with cos^2(theta/2)
+//   also from the FP64 trace, so it stays accurate up to a half turn);
+//   element lengths come
 //   from node differences rounded to FP32, which the neighbour that needs the
 //   same length receives instead of recomputing it.  The FP32 constants are
 //   held in an LDS table for the whole launch (1/m, 1/J, 1/l0, 1/D0 formed
@@ -185,9 +188,9 @@ namespace erod {
 
     // FP32-bulk: skew part and trace in FP64, prefactor in FP32.  With
     // a = sin^2(theta/2) (3 - tr)/4 + 5e-11, the reference prefactor
-    // theta / (2 sin theta) equals asin(sqrt a) / (2 sqrt(a) sqrt(1 - a)); the
-    // first factor is the series 1 + a/6 + 3a^2/40 + 5a^3/112 + 35a^4/1152
-    // for a < 0.02 (theta < 16 degrees) and the library asin above.
+    // theta / (2 sin theta) equals asin(sqrt a) / (2 sqrt(a) sqrt(1 - a)),
+    // taken as a series for a < 0.02 (theta < 16 degrees) and with library
+    // asin above, 1 - a coming from the FP64 trace.
     __device__ __forceinline__ double dot3_fma(const double* a, const double* b) {
       return __builtin_fma(a[2], b[2], __builtin_fma(a[1], b[1], a[0] * b[0]));
     }
@@ -209,7 +212,7 @@ namespace erod {
       const double s1 = skew3_fma(qc + 0, qn + 6, qc + 6, qn + 0);
       const double s2 = skew3_fma(qc + 3, qn + 0, qc + 0, qn + 3);
       float a = static_cast<float>((3.0 - tr) * 0.25) + 5e-11f;
-      a = fminf(fmaxf(a, 0.0f), 0.999999f);
+      a = fminf(fmaxf(a, 0.0f), 1.0f);
       // st = (theta / 2) / sin(theta) / D0 with a = sin^2(theta / 2):
       // theta / (2 sin theta) = g(a) / 2, g(a) = asin(sqrt a) / (sqrt(a) sqrt(1 - a))
       //                      = sum_n 4^n (n!)^2 / (2n + 1)! a^n = 1 + 2a/3 + 8a^2/15 + 16a^3/35 + 128a^4/315 + ...
@@ -218,8 +221,15 @@ namespace erod {
       if (a < 0.02f) {
         g = madd(a, madd(a, madd(a, madd(a, 128.0f / 315.0f, 16.0f / 35.0f), 8.0f / 15.0f), 2.0f / 3.0f), 1.0f);
       } else {
-        const float sh = sqrt_(a);
-        g = asinf(sh) / (sh * sqrt_(1.0f - a));
+        // 1 - a = cos^2(theta / 2) - 5e-11 = (1 + tr) / 4 - 5e-11 from the FP64
+        // trace, and theta / 2 = asin(sqrt a) up to a quarter turn, pi/2 -
+        // asin(sqrt(1 - a)) beyond: accurate up to a relative rotation of pi,
+        // where 1 - a formed in FP32 would cancel (the floor only keeps an
+        // exact half turn, where the axis is undefined, finite)
+        const float b = fmaxf(static_cast<float>((1.0 + tr) * 0.25) - 5e-11f, 1e-30f);
+        const float sh = sqrt_(a), cb = sqrt_(b);
+        const float half = a <= 0.5f ? asinf(sh) : 1.57079632679489662f - asinf(cb);
+        g = half / (sh * cb);
       }
       const float st = 0.5f * g * iD0;
       kap[0] = st * static_cast<float>(s0);

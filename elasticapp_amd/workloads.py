@@ -20,7 +20,8 @@ equal.  The reference's timestep rule dt = 0.01 L / n
 wave speed sqrt(E / rho) = sqrt(1e6 / 5e3); for other materials the same
 Courant number is kept: dt_i = 0.01 (L_i / n_i) sqrt(1e6 / 5e3) / sqrt(E_i / rho_i),
 and a batch uses the smallest dt of its rods (the reference steps every
-system with one dt).
+system with one dt).  The ragged batch also carries each rod's own dt
+(`info["rod_dt"]`) for the fused paths, which step every rod with its own.
 
 A rod stepped with its own dt (the serving library) must also respect two
 limits of the explicit step that the axial rule does not see when the radius
