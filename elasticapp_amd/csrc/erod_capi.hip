@@ -152,8 +152,9 @@ int erod_field_shape(int field, int* ncomp, int* stagger) {
 }
 
 int erod_batch_create(const uint64_t* n_elems, uint64_t n_rods, int damped, erod_batch** out) {
-  if (!out || !n_elems || n_rods == 0) {
-    set_error("erod_batch_create: need at least one rod");
+  if (!out || !n_elems || n_rods == 0 || n_rods > UINT32_MAX) {
+    // rod ids travel as 32-bit values in the tile tables and record uploads
+    set_error("erod_batch_create: need between 1 and 2^32 - 1 rods");
     return EROD_ERR_INVALID;
   }
   *out = nullptr;
@@ -167,8 +168,10 @@ int erod_batch_create(const uint64_t* n_elems, uint64_t n_rods, int damped, erod
     set_error("no HIP device visible");
     return EROD_ERR_NO_DEVICE;
   }
+  int device = 0;
+  EROD_HIP(hipGetDevice(&device));
   auto* b = new erod_batch();
-  EROD_HIP(hipGetDevice(&b->device));
+  b->device = device;
   b->n_rods = n_rods;
   b->damped = damped != 0;
   b->n_elems.assign(n_elems, n_elems + n_rods);
@@ -244,19 +247,19 @@ int erod_batch_destroy(erod_batch* b) {
 }
 
 int erod_batch_device(const erod_batch* b, int* device) {
-  if (check_batch(b)) return EROD_ERR_INVALID;
+  if (check_batch(b) || !device) return EROD_ERR_INVALID;
   *device = b->device;
   return EROD_OK;
 }
 
 int erod_batch_slots(const erod_batch* b, uint64_t* element_slots) {
-  if (check_batch(b)) return EROD_ERR_INVALID;
+  if (check_batch(b) || !element_slots) return EROD_ERR_INVALID;
   *element_slots = b->E;
   return EROD_OK;
 }
 
 int erod_rod_start(const erod_batch* b, uint64_t rod, uint64_t* start) {
-  if (check_batch(b)) return EROD_ERR_INVALID;
+  if (check_batch(b) || !start) return EROD_ERR_INVALID;
   if (rod >= b->n_rods) {
     set_error("rod index out of range");
     return EROD_ERR_INVALID;
@@ -365,6 +368,11 @@ int erod_add_end_force(erod_batch* b, uint64_t rod, const double F[3], int ramp,
     set_error("erod_add_end_force: bad argument");
     return EROD_ERR_INVALID;
   }
+  // a ramp divides by its interval: an empty one would give 0/0 at t = start
+  if (ramp != EROD_RAMP_NONE && !(interval > 0.0 && std::isfinite(interval) && std::isfinite(start))) {
+    set_error("erod_add_end_force: a ramped force needs a positive, finite interval and a finite start");
+    return EROD_ERR_INVALID;
+  }
   RodInfo& ri = b->rods[rod];
   ri.flags |= 2u;
   std::memcpy(ri.force, F, sizeof(ri.force));
@@ -419,7 +427,10 @@ int erod_apply_post_dynamic(erod_batch* b, double time) {
 }
 
 int erod_step(erod_batch* b, double* time, double dt, uint64_t n) {
-  if (!time) return EROD_ERR_INVALID;
+  if (!time) {
+    set_error("erod_step: null time");
+    return EROD_ERR_INVALID;
+  }
   if (int rc = check_plain(b)) return rc;
   int rc = sync_rods(b);
   if (rc) return rc;
@@ -442,13 +453,13 @@ int erod_step(erod_batch* b, double* time, double dt, uint64_t n) {
 
 int erod_set_rod_dt(erod_batch* b, const double* dt_per_rod) {
   if (check_batch(b) || !dt_per_rod) return EROD_ERR_INVALID;
-  for (uint64_t r = 0; r < b->n_rods; ++r) {
+  // all or nothing: a rejected call leaves every rod's dt as it was
+  for (uint64_t r = 0; r < b->n_rods; ++r)
     if (!(dt_per_rod[r] > 0.0) || !std::isfinite(dt_per_rod[r])) {
       set_error("erod_set_rod_dt: dt must be positive and finite");
       return EROD_ERR_INVALID;
     }
-    b->rods[r].dt = dt_per_rod[r];
-  }
+  for (uint64_t r = 0; r < b->n_rods; ++r) b->rods[r].dt = dt_per_rod[r];
   b->rods_dirty = true;
   return EROD_OK;
 }
@@ -669,10 +680,16 @@ int erod_step_fused_graph(erod_batch* b, int precision, uint64_t n, uint64_t chu
       ++launches;
     }
     hipError_t e = hipStreamEndCapture(b->stream, &g);
-    if (lrc) return lrc;
-    EROD_HIP(e);
-    EROD_HIP(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+    if (lrc || e != hipSuccess) {
+      if (g) hipGraphDestroy(g);
+      return lrc ? lrc : hip_check(e, "hipStreamEndCapture");
+    }
+    e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      b->graph_exec = nullptr;
+      return hip_check(e, "hipGraphInstantiate");
+    }
     b->graph_precision = precision;
     b->graph_chunk = chunk;
     b->graph_launches = launches;

@@ -140,7 +140,11 @@ namespace erod {
     // FP32-bulk: the same map written in u itself,
     //   q' = q - th^2 B q + B (u . q) u + A (q x u),
     // A = sin(th)/th, B = (1 - cos(th))/th^2, both even series in th^2 = u . u
-    // (exact to rounding for th < 1e-2), so no sqrt or division per step.
+    // (exact to rounding for th < 1e-2), so no sqrt or division per step.  Each
+    // column c of q (the components of d1, d2, d3 along one axis) maps as
+    // c' = M c with M = (1 - th^2 B) I + B u u^T + A [c -> c x u]: M is formed
+    // once (15 operations) and applied with 27 FMAs, against 57 operations for
+    // the three columns taken one at a time.
     template <>
     __device__ __forceinline__ void rodrigues<float>(double q[9], const double w[3], double cdt) {
       const double u0 = cdt * w[0], u1 = cdt * w[1], u2 = cdt * w[2];
@@ -159,13 +163,17 @@ namespace erod {
         B = 2.0 * sh * sh / t2;
       }
       const double cq = __builtin_fma(-t2, B, 1.0);
+      const double b0 = B * u0, b1 = B * u1, b2 = B * u2, a0 = A * u0, a1 = A * u1, a2 = A * u2;
+      const double m00 = __builtin_fma(b0, u0, cq), m11 = __builtin_fma(b1, u1, cq), m22 = __builtin_fma(b2, u2, cq);
+      const double m01 = __builtin_fma(b0, u1, a2), m10 = __builtin_fma(b1, u0, -a2);
+      const double m02 = __builtin_fma(b0, u2, -a1), m20 = __builtin_fma(b2, u0, a1);
+      const double m12 = __builtin_fma(b1, u2, a0), m21 = __builtin_fma(b2, u1, -a0);
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         const double a = q[j], b = q[3 + j], cc = q[6 + j];
-        const double bu = B * __builtin_fma(cc, u2, __builtin_fma(b, u1, a * u0));
-        q[j] = __builtin_fma(cq, a, __builtin_fma(bu, u0, A * __builtin_fma(b, u2, -(cc * u1))));
-        q[3 + j] = __builtin_fma(cq, b, __builtin_fma(bu, u1, A * __builtin_fma(cc, u0, -(a * u2))));
-        q[6 + j] = __builtin_fma(cq, cc, __builtin_fma(bu, u2, A * __builtin_fma(a, u1, -(b * u0))));
+        q[j] = __builtin_fma(m02, cc, __builtin_fma(m01, b, m00 * a));
+        q[3 + j] = __builtin_fma(m12, cc, __builtin_fma(m11, b, m10 * a));
+        q[6 + j] = __builtin_fma(m22, cc, __builtin_fma(m21, b, m20 * a));
       }
     }
 

@@ -189,6 +189,36 @@ def test_edge_lengths_are_independent_of_placement(precision):
         a.step_fused(1, precision)
 
 
+def test_rejected_arguments_leave_the_batch_as_it_was():
+    """Bad arguments raise and change nothing: a ramped force with an empty
+    interval (its ramp would be 0/0 at the start time), a rod dt that is not
+    positive (the call is all or nothing), a graph length that is not a whole
+    number of chunks.  The batch then steps bitwise like an untouched one."""
+    wl = random_workload(4, 8, 40, seed=71, steps=50)
+    a, b = device_block(wl), device_block(wl)
+    dts = np.asarray([wl.dt * (1.0 - 0.1 * r) for r in range(4)])
+    for blk in (a, b):
+        blk.set_rod_dt(dts)
+    with pytest.raises(ErodError, match="interval"):
+        a.add_end_force(0, [1.0, 0.0, 0.0], ramp=1, start=0.0, interval=0.0)
+    with pytest.raises(ErodError, match="interval"):
+        a.add_end_force(0, [1.0, 0.0, 0.0], ramp=2, start=0.0, interval=float("nan"))
+    bad = dts.copy()
+    bad[2] = -1.0
+    with pytest.raises(ErodError, match="positive"):
+        a.set_rod_dt(bad)
+    with pytest.raises(ErodError, match="multiple"):
+        a.step_fused_graph(30, 20, FP64)
+    for blk in (a, b):
+        blk.step_fused(wl.steps, FP64)
+    for r in range(4):
+        for t in STATE:
+            assert np.array_equal(a.get_rod(t, r), b.get_rod(t, r)), (r, t)
+    assert np.array_equal(a.rod_time(), b.rod_time())
+    a.close()
+    b.close()
+
+
 def test_autotune_keeps_state_and_picks_a_shape():
     wl = random_workload(25, 4, 200, seed=31, steps=30)
     a = device_block(wl)
