@@ -82,6 +82,7 @@ def load_library():
         "erod_add_end_force": (i32, [vp, u64, P(dbl), i32, dbl, dbl]),
         "erod_set_gravity": (i32, [vp, P(dbl)]),
         "erod_set_plane_contact": (i32, [vp, dbl, dbl, dbl, dbl, dbl, dbl]),
+        "erod_set_plane_normal": (i32, [vp, P(dbl)]),
         "erod_apply_post_kinematic": (i32, [vp, dbl]),
         "erod_apply_pre_dynamic": (i32, [vp, dbl]),
         "erod_apply_post_dynamic": (i32, [vp, dbl]),
@@ -261,8 +262,12 @@ class DeviceBlock:
         g = np.ascontiguousarray(g, dtype=np.float64).reshape(3)
         _check(_LIB.erod_set_gravity(self._h, _dp(g)))
 
-    def set_plane_contact(self, k, c, mu, ct, z0=0.0, offset=0.0):
+    def set_plane_contact(self, k, c, mu, ct, z0=0.0, offset=0.0, normal=None):
+        """Per-node contact with the plane normal . x = z0 (normal default (0, 0, 1))."""
         _check(_LIB.erod_set_plane_contact(self._h, k, c, mu, ct, z0, offset))
+        if normal is not None:
+            n = np.ascontiguousarray(normal, dtype=np.float64).reshape(3)
+            _check(_LIB.erod_set_plane_normal(self._h, _dp(n)))
 
     def apply_post_kinematic(self, t):
         _check(_LIB.erod_apply_post_kinematic(self._h, float(t)))

@@ -23,7 +23,7 @@ def read_header(path):
         if f.read(8) != MAGIC:
             raise ValueError("not an erod checkpoint: %s" % path)
         version, damped, n_rods, _ = struct.unpack("<IIQQ", f.read(24))
-        if version != 1:
+        if version not in (1, 2):
             raise ValueError("unsupported checkpoint version %d" % version)
         raw = f.read(8 * n_rods)
         if len(raw) != 8 * n_rods:
@@ -41,6 +41,8 @@ def read(path):
     o = 8
     version, damped, n_rods, slots = struct.unpack_from("<IIQQ", data, o)
     o += 24
+    if version not in (1, 2):
+        raise ValueError("unsupported checkpoint version %d" % version)
 
     def arr(dtype, n):
         nonlocal o
@@ -69,10 +71,15 @@ def read(path):
     o += 24
     contact, = struct.unpack_from("<i", data, o)
     o += 4
-    plane = struct.unpack_from("<6d", data, o)
+    plane = dict(zip(("k", "c", "mu", "ct", "z0", "offset"), struct.unpack_from("<6d", data, o)))
     o += 48
-    out["env"] = {"gravity": bool(grav), "g": np.array(g), "contact": bool(contact),
-                  "plane": dict(zip(("k", "c", "mu", "ct", "z0", "offset"), plane))}
+    # version 2 adds the plane's unit normal; version 1 planes are z = z0
+    if version >= 2:
+        plane["normal"] = np.array(struct.unpack_from("<3d", data, o))
+        o += 24
+    else:
+        plane["normal"] = np.array([0.0, 0.0, 1.0])
+    out["env"] = {"gravity": bool(grav), "g": np.array(g), "contact": bool(contact), "plane": plane}
     nf, = struct.unpack_from("<I", data, o)
     o += 4
     fields = {}

@@ -406,6 +406,19 @@ int erod_set_plane_contact(erod_batch* b, double k, double c, double mu, double 
   return EROD_OK;
 }
 
+int erod_set_plane_normal(erod_batch* b, const double n[3]) {
+  if (check_batch(b) || !n) return EROD_ERR_INVALID;
+  // scaled to unit length; (0, 0, 1) and the other axes stay exact
+  const double len = std::sqrt((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2]);
+  if (!(len > 0.0) || !std::isfinite(len)) {
+    set_error("erod_set_plane_normal: the normal must be finite and non-zero");
+    return EROD_ERR_INVALID;
+  }
+  for (int c = 0; c < 3; ++c) b->env.n[c] = n[c] / len;
+  b->graph_precision = -1;
+  return EROD_OK;
+}
+
 int erod_apply_post_kinematic(erod_batch* b, double time) {
   (void)time;
   if (int rc = check_plain(b)) return rc;

@@ -71,6 +71,7 @@ namespace erod {
     double g[3];
     int contact;
     double k, c, mu, ct, z0, offset;
+    double n[3] = {0.0, 0.0, 1.0};  // unit normal of the contact plane n . x = z0
   };
 
   // A tile: the rods one workgroup of the fused kernel owns.

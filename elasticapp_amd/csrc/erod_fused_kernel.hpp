@@ -435,17 +435,22 @@ namespace erod {
       if (env.gravity)
         for (int c = 0; c < 3; ++c) fe[c] += env.g[c] * m;
       if (env.contact) {
-        const double gap = x[2] - env.z0 - env.offset;
+        const double gap = ((env.n[0] * x[0] + env.n[1] * x[1]) + env.n[2] * x[2]) - env.z0 - env.offset;
         if (gap < 0.0) {
           const double delta = -gap;
-          const double vn = -v[2];
+          const double vnv = (env.n[0] * v[0] + env.n[1] * v[1]) + env.n[2] * v[2];
+          const double vn = -vnv;
           const double fn = fmax(0.0, env.k * delta + env.c * vn);
-          const double vtn = sqrt(v[0] * v[0] + v[1] * v[1]);
-          fe[2] += fn;
+          double vt[3];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) vt[c] = v[c] - vnv * env.n[c];
+          const double vtn = sqrt((vt[0] * vt[0] + vt[1] * vt[1]) + vt[2] * vt[2]);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) fe[c] += fn * env.n[c];
           if (vtn > 0.0) {
             const double ft = fmin(env.ct * vtn, env.mu * fn);
-            fe[0] -= ft * (v[0] / vtn);
-            fe[1] -= ft * (v[1] / vtn);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) fe[c] -= ft * (vt[c] / vtn);
           }
         }
       }

@@ -9,12 +9,13 @@
 // environment), so a run resumed from it continues bit for bit.
 //
 // Layout (little-endian, no padding):
-//   "ERODCKP1"  u32 version (1)  u32 damped  u64 n_rods  u64 element_slots
+//   "ERODCKP1"  u32 version (2)  u32 damped  u64 n_rods  u64 element_slots
 //   u64 n_elems[n_rods]  u64 n_active[n_rods]  u32 steps_left[n_rods]  u8 budgets
 //   f64 rod_time[n_rods]
 //   per rod: f64 dt, gamma_f, gamma_t  u32 flags  f64 clamp_x[3], clamp_q[9], force[3]
 //            i32 ramp  f64 ramp_start, ramp_interval
-//   i32 gravity  f64 g[3]  i32 contact  f64 k, c, mu, ct, z0, offset
+//   i32 gravity  f64 g[3]  i32 contact  f64 k, c, mu, ct, z0, offset  f64 n[3]
+//   (version 1 files, without the plane normal n, load with n = (0, 0, 1))
 //   u32 n_fields; per field: u32 id  u32 ncomp  u64 count  f64 data[ncomp * count]
 //   "ERODEND1"
 #include <cstdio>
@@ -30,7 +31,7 @@ using namespace erod;
 namespace {
   constexpr char kMagic[9] = "ERODCKP1";
   constexpr char kEnd[9] = "ERODEND1";
-  constexpr uint32_t kVersion = 1;
+  constexpr uint32_t kVersion = 2;
 
   struct File {
     std::FILE* f = nullptr;
@@ -75,7 +76,8 @@ int erod_save(erod_batch* b, const char* path) {
   }
   const Env& e = b->env;
   ok = ok && put(f, int32_t(e.gravity)) && put_n(f, e.g, 3) && put(f, int32_t(e.contact)) && put(f, e.k) && put(f, e.c) &&
-       put(f, e.mu) && put(f, e.ct) && put(f, e.z0) && put(f, e.offset) && put(f, uint32_t(EROD_N_FIELDS));
+       put(f, e.mu) && put(f, e.ct) && put(f, e.z0) && put(f, e.offset) && put_n(f, e.n, 3) &&
+       put(f, uint32_t(EROD_N_FIELDS));
   std::vector<double> buf;
   for (int fid = 0; ok && fid < EROD_N_FIELDS; ++fid) {
     const uint32_t nc = static_cast<uint32_t>(field_info(fid).ncomp);
@@ -99,9 +101,9 @@ int erod_load(const char* path, erod_batch** out) {
   char magic[8];
   uint32_t version = 0, damped = 0;
   uint64_t n_rods = 0, E = 0;
-  if (!get_n(f, magic, 8) || std::memcmp(magic, kMagic, 8) != 0 || !get(f, version) || version != kVersion ||
-      !get(f, damped) || !get(f, n_rods) || !get(f, E) || n_rods == 0 || n_rods > (uint64_t(1) << 32))
-    return io_error(std::string("erod_load: not an erod checkpoint (version 1): ") + path);
+  if (!get_n(f, magic, 8) || std::memcmp(magic, kMagic, 8) != 0 || !get(f, version) || version < 1 ||
+      version > kVersion || !get(f, damped) || !get(f, n_rods) || !get(f, E) || n_rods == 0 || n_rods > UINT32_MAX)
+    return io_error(std::string("erod_load: not an erod checkpoint (version 1 or 2): ") + path);
   // every rod takes at least its header entries and its record: refuse a count
   // the file cannot hold before sizing anything by it
   constexpr uint64_t kRodBytes = 8 + 8 + 4 + 8 + 3 * 8 + 4 + 15 * 8 + 4 + 2 * 8;
@@ -136,7 +138,7 @@ int erod_load(const char* path, erod_batch** out) {
   int32_t grav = 0, cont = 0;
   uint32_t nf = 0;
   ok = ok && get(f, grav) && get_n(f, e.g, 3) && get(f, cont) && get(f, e.k) && get(f, e.c) && get(f, e.mu) && get(f, e.ct) &&
-       get(f, e.z0) && get(f, e.offset) && get(f, nf) && nf == EROD_N_FIELDS;
+       get(f, e.z0) && get(f, e.offset) && (version < 2 || get_n(f, e.n, 3)) && get(f, nf) && nf == EROD_N_FIELDS;
   e.gravity = grav;
   e.contact = cont;
   if (!ok) return io_error("erod_load: bad rod records or environment");

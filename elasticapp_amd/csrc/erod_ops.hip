@@ -340,7 +340,7 @@ namespace erod {
       }
     }
 
-    // gravity and per-node plane contact on every real node
+    // flags the ghost node between two rods
     __global__ void k_mark_ghost_nodes(uint8_t* __restrict__ flag, const uint64_t* __restrict__ ghost, uint64_t n) {
       GRID_STRIDE(i, n) flag[ghost[i]] = 1;
     }
@@ -355,18 +355,20 @@ namespace erod {
         if (env.gravity)
           for (int c = 0; c < 3; ++c) fe[c * N + i] += env.g[c] * m[i];
         if (env.contact) {
-          const double gap = x[2 * N + i] - env.z0 - env.offset;
+          // the plane n . x = z0; the node's height above it, less the offset
+          const double gap = ((env.n[0] * x[i] + env.n[1] * x[N + i]) + env.n[2] * x[2 * N + i]) - env.z0 - env.offset;
           if (gap < 0.0 && !is_ghost[i]) {
             const double delta = -gap;
-            const double vn = -v[2 * N + i];
+            const double vnv = (env.n[0] * v[i] + env.n[1] * v[N + i]) + env.n[2] * v[2 * N + i];
+            const double vn = -vnv;  // approach speed along the normal
             const double fn = fmax(0.0, env.k * delta + env.c * vn);
-            const double vt0 = v[i], vt1 = v[N + i];
-            const double vtn = sqrt(vt0 * vt0 + vt1 * vt1);
-            fe[2 * N + i] += fn;
+            double vt[3];
+            for (int c = 0; c < 3; ++c) vt[c] = v[c * N + i] - vnv * env.n[c];
+            const double vtn = sqrt((vt[0] * vt[0] + vt[1] * vt[1]) + vt[2] * vt[2]);
+            for (int c = 0; c < 3; ++c) fe[c * N + i] += fn * env.n[c];
             if (vtn > 0.0) {
               const double ft = fmin(env.ct * vtn, env.mu * fn);
-              fe[i] -= ft * (vt0 / vtn);
-              fe[N + i] -= ft * (vt1 / vtn);
+              for (int c = 0; c < 3; ++c) fe[c * N + i] -= ft * (vt[c] / vtn);
             }
           }
         }

@@ -117,6 +117,13 @@ namespace erod {
                    double interval = 1.0) {
       check(erod_add_end_force(b_, rod, F, ramp, start, interval));
     }
+    void gravity(const double g[3]) { check(erod_set_gravity(b_, g)); }
+    // per-node contact with the plane n . x = z0 (n = (0, 0, 1) unless given)
+    void plane_contact(double k, double c, double mu, double ct, double z0 = 0.0, double offset = 0.0,
+                       const double* normal = nullptr) {
+      check(erod_set_plane_contact(b_, k, c, mu, ct, z0, offset));
+      if (normal) check(erod_set_plane_normal(b_, normal));
+    }
     void post_kinematic(double t) { check(erod_apply_post_kinematic(b_, t)); }
     void pre_dynamic(double t) { check(erod_apply_pre_dynamic(b_, t)); }
     void post_dynamic(double t) { check(erod_apply_post_dynamic(b_, t)); }

@@ -50,7 +50,7 @@ class Workload:
     clamps: list = field(default_factory=list)   # (rod, x[3], Q[9])
     forces: list = field(default_factory=list)   # (rod, F[3], ramp, start, interval)
     gravity: tuple = None
-    contact: tuple = None                        # (k, c, mu, ct, z0, offset)
+    contact: tuple = None                        # (k, c, mu, ct, z0, offset[, normal[3]])
     info: dict = field(default_factory=dict)
 
     @property

@@ -122,10 +122,16 @@ int erod_add_clamp(erod_batch* b, uint64_t rod, uint64_t index, const double x[3
 int erod_add_end_force(erod_batch* b, uint64_t rod, const double F[3], int ramp, double start, double interval);
 /* ExternalLoads += g (x) Mass (GravityField.hpp:239-246). */
 int erod_set_gravity(erod_batch* b, const double g[3]);
-/* Per-node plane contact z = z0 + offset with a linear spring-dashpot normal
- * force max(0, k delta + c v_n) and friction min(ct |v_t|, mu f_N) against
- * the slip (LinearSpringDashpot.cpp:49-71 applied per node). */
+/* Per-node plane contact: a node whose height n . x above the plane
+ * n . x = z0 is below offset gets a linear spring-dashpot normal force
+ * max(0, k delta + c v_n) along n and friction min(ct |v_t|, mu f_N) against
+ * its slip in the plane (LinearSpringDashpot.cpp:49-71 applied per node).
+ * The normal n is (0, 0, 1) unless erod_set_plane_normal sets it. */
 int erod_set_plane_contact(erod_batch* b, double k, double c, double mu, double ct, double z0, double offset);
+/* Unit normal of the contact plane (PlanePrimitive's normal, distance along it
+ * as in CapsulePlane.hpp:73-80); scaled to unit length, zero or non-finite
+ * normals are refused. */
+int erod_set_plane_normal(erod_batch* b, const double n[3]);
 int erod_apply_post_kinematic(erod_batch* b, double time);
 int erod_apply_pre_dynamic(erod_batch* b, double time);
 int erod_apply_post_dynamic(erod_batch* b, double time);

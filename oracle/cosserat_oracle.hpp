@@ -32,7 +32,9 @@
 //     straight unstressed rod, axial stretching tip displacement
 //     F L / (E A - F), first-order convergence of the Timoshenko cantilever
 //     deflection, exact free fall under gravity, a resting contact carrying
-//     the weight, sliding friction decelerating at mu g (damped closed form);
+//     the weight, sliding friction decelerating at mu g (damped closed form),
+//     and contact on a turned plane equal to the turned level case (the plane
+//     normal in the gap, approach speed, slip and both forces);
 //   * a rod stepped inside a multi-rod block equals the same rod alone, bitwise.
 #pragma once
 
@@ -555,6 +557,7 @@ namespace oracle {
   };
   struct PlaneContact {  // per-node plane contact with regularised Coulomb friction
     double k, c, mu, ct, z0, offset;
+    double n[3] = {0.0, 0.0, 1.0};  // unit normal of the plane n . x = z0
   };
 
   inline double ramp_value(int kind, double t, double start, double interval) {
@@ -620,18 +623,23 @@ namespace oracle {
       for (std::size_t r = 0; r < b.n_elems.size(); ++r) {
         const std::size_t st = b.starts[r];
         for (std::size_t i = st; i <= st + b.n_elems[r]; ++i) {
-          const T gap = x[2 * N + i] - T(s.plane.z0) - T(s.plane.offset);
+          // the plane's normal and contact law as in CapsulePlane.hpp:73-80
+          // (distance along the plane normal) and LinearSpringDashpot.cpp:49-71
+          // (normal and tangential relative velocity, friction capped at mu fN)
+          const T nx = T(s.plane.n[0]), ny = T(s.plane.n[1]), nz = T(s.plane.n[2]);
+          const T gap = ((nx * x[i] + ny * x[N + i]) + nz * x[2 * N + i]) - T(s.plane.z0) - T(s.plane.offset);
           if (!(gap < T(0))) continue;
           const T delta = -gap;
-          const T vn = -v[2 * N + i];  // approach speed along the plane normal
+          const T vnv = (nx * v[i] + ny * v[N + i]) + nz * v[2 * N + i];
+          const T vn = -vnv;  // approach speed along the plane normal
           const T fn = std::max(T(0), T(s.plane.k) * delta + T(s.plane.c) * vn);
-          const T vt0 = v[i], vt1 = v[N + i];
-          const T vtn = std::sqrt(vt0 * vt0 + vt1 * vt1);
-          fe[2 * N + i] += fn;
+          const T vt[3] = {v[i] - vnv * nx, v[N + i] - vnv * ny, v[2 * N + i] - vnv * nz};
+          const T vtn = std::sqrt((vt[0] * vt[0] + vt[1] * vt[1]) + vt[2] * vt[2]);
+          const T nv[3] = {nx, ny, nz};
+          for (int c = 0; c < 3; ++c) fe[c * N + i] += fn * nv[c];
           if (vtn > T(0)) {
             const T ft = std::min(T(s.plane.ct) * vtn, T(s.plane.mu) * fn);
-            fe[i] -= ft * (vt0 / vtn);
-            fe[N + i] -= ft * (vt1 / vtn);
+            for (int c = 0; c < 3; ++c) fe[c * N + i] -= ft * (vt[c] / vtn);
           }
         }
       }

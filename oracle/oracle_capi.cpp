@@ -7,6 +7,7 @@
 // 291-321) plus hook descriptors; steps can be spread over a std::thread pool,
 // one task per block as tbb::parallel_for_each does (elastica/Parallel/for_each.hpp:90-99).
 #include <atomic>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <thread>
@@ -174,7 +175,23 @@ int oracle_set_gravity(void* h, const double* g3) {
 int oracle_set_plane_contact(void* h, double k, double c, double mu, double ct, double z0, double offset) {
   auto* s = static_cast<Sys*>(h);
   s->contact = true;
-  s->plane = oracle::PlaneContact{k, c, mu, ct, z0, offset};
+  // the plane's normal stays as set (oracle_set_plane_normal)
+  s->plane.k = k;
+  s->plane.c = c;
+  s->plane.mu = mu;
+  s->plane.ct = ct;
+  s->plane.z0 = z0;
+  s->plane.offset = offset;
+  return 0;
+}
+
+// unit normal of the contact plane n . x = z0 (default (0, 0, 1)), scaled to
+// unit length as the device library does; 1 if n is zero or not finite
+int oracle_set_plane_normal(void* h, const double* n3) {
+  auto* s = static_cast<Sys*>(h);
+  const double len = std::sqrt((n3[0] * n3[0] + n3[1] * n3[1]) + n3[2] * n3[2]);
+  if (!(len > 0.0) || !std::isfinite(len)) return 1;
+  for (int c = 0; c < 3; ++c) s->plane.n[c] = n3[c] / len;
   return 0;
 }
 

@@ -61,6 +61,7 @@ def _lib(path=None):
         L.oracle_add_end_force.argtypes = [vp, u64, dp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.oracle_set_gravity.argtypes = [vp, dp]
         L.oracle_set_plane_contact.argtypes = [vp] + [ctypes.c_double] * 6
+        L.oracle_set_plane_normal.argtypes = [vp, dp]
         L.oracle_time.restype = ctypes.c_double
         L.oracle_time.argtypes = [vp]
         L.oracle_set_time.argtypes = [vp, ctypes.c_double]
@@ -171,8 +172,13 @@ class System:
         g = np.ascontiguousarray(g, dtype=np.float64).reshape(3)
         self._L.oracle_set_gravity(self._h, _dptr(g))
 
-    def set_plane_contact(self, k, c, mu, ct, z0=0.0, offset=0.0):
+    def set_plane_contact(self, k, c, mu, ct, z0=0.0, offset=0.0, normal=None):
+        """Contact with the plane normal . x = z0 (normal default (0, 0, 1))."""
         self._L.oracle_set_plane_contact(self._h, k, c, mu, ct, z0, offset)
+        if normal is not None:
+            n = np.ascontiguousarray(normal, dtype=np.float64).reshape(3)
+            if self._L.oracle_set_plane_normal(self._h, _dptr(n)) != 0:
+                raise ValueError("plane normal must be finite and non-zero")
 
     @property
     def time(self):

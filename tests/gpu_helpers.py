@@ -9,6 +9,34 @@ from oracle import pyoracle
 STATE = ("Position", "Velocity", "Director", "AngularVelocity")
 
 
+def _turn_directors(d, R):
+    # director lines 3 p + r hold Q(p, r); each director row d_p turns to R d_p
+    return (d.T.reshape(-1, 3, 3) @ R.T).reshape(-1, 9).T.copy()
+
+
+def rotated(wl, R):
+    """The workload turned by the rotation R: world-frame quantities
+    (positions, velocities, director rows, gravity, end forces, clamp poses,
+    the contact plane's normal) rotate; material-frame ones (angular
+    velocities, rest strains and curvatures, constants) do not."""
+    states = []
+    for st in wl.states:
+        s = dict(st)
+        s["Position"] = R @ st["Position"]
+        s["Velocity"] = R @ st["Velocity"]
+        s["Director"] = _turn_directors(st["Director"], R)
+        states.append(s)
+    clamps = [(r, R @ x, _turn_directors(np.asarray(Q).reshape(9, 1), R)[:, 0]) for r, x, Q in wl.clamps]
+    forces = [(r, R @ np.asarray(F), ramp, t0, T) for r, F, ramp, t0, T in wl.forces]
+    gravity = None if wl.gravity is None else tuple(R @ np.asarray(wl.gravity))
+    contact = None
+    if wl.contact is not None:
+        n = np.asarray(wl.contact[6]) if len(wl.contact) > 6 else np.array([0.0, 0.0, 1.0])
+        contact = tuple(wl.contact[:6]) + (tuple(R @ n),)
+    return Workload(wl.name + "-rotated", list(wl.n_elems), states, wl.dt, wl.steps, clamps=clamps, forces=forces,
+                    gravity=gravity, contact=contact, info=dict(wl.info))
+
+
 def random_workload(n_rods, n_min, n_max, seed, steps=100, forces=True, kappa=0.5, vstd=1e-2, lengths=None):
     """Randomly bent, moving, loaded rods; `lengths` fixes the element counts
     (n_rods, n_min and n_max are then unused)."""

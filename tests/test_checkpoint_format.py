@@ -11,18 +11,20 @@ import elasticapp_amd as ea
 from elasticapp_amd import checkpoint
 
 
-def _pack(n_elems):
+def _pack(n_elems, version=2):
     ea.load_library()
     n_rods = len(n_elems)
     E = sum(n_elems) + 2 * n_rods - 2
     b = bytearray(b"ERODCKP1")
-    b += struct.pack("<IIQQ", 1, 1, n_rods, E)
+    b += struct.pack("<IIQQ", version, 1, n_rods, E)
     b += np.array(n_elems, "<u8").tobytes() + np.array(n_elems, "<u8").tobytes()
     b += np.array([7] * n_rods, "<u4").tobytes() + b"\x01"
     b += np.arange(n_rods, dtype="<f8").tobytes()
     for r in range(n_rods):
         b += struct.pack("<3dI3d9d3di2d", 1e-3 * (r + 1), 0.1, 0.2, 2, 0, 0, 0, *([0.0] * 9), 1.0, 2.0, 3.0, 1, 0.0, 5.0)
     b += struct.pack("<i3di6d", 1, 0.0, 0.0, -9.81, 0, 0, 0, 0, 0, 0, 0)
+    if version >= 2:
+        b += struct.pack("<3d", 0.6, 0.0, 0.8)
     b += struct.pack("<I", len(ea.FIELDS))
     fields = {}
     for name, (fid, nc, st) in sorted(ea.FIELDS.items(), key=lambda kv: kv[1][0]):
@@ -43,10 +45,20 @@ def test_reader_parses_the_documented_layout(tmp_path):
     assert list(ck["starts"]) == [0, 7, 12] and ck["budgets"] and list(ck["steps_left"]) == [7, 7, 7]
     assert ck["rods"][2]["dt"] == 3e-3 and list(ck["rods"][1]["force"]) == [1.0, 2.0, 3.0] and ck["rods"][0]["ramp"] == 1
     assert ck["env"]["gravity"] and not ck["env"]["contact"] and ck["env"]["g"][2] == -9.81
+    assert list(ck["env"]["plane"]["normal"]) == [0.6, 0.0, 0.8]
     for name, a in fields.items():
         assert np.array_equal(ck["fields"][name], a), name
     x = checkpoint.rod_field(ck, "Position", 1)
     assert np.array_equal(x, fields["Position"][:, 7:11])
+
+
+def test_reader_reads_version_one_with_the_horizontal_plane(tmp_path):
+    data, fields = _pack([5, 3], version=1)
+    p = tmp_path / "v1.ckpt"
+    p.write_bytes(data)
+    ck = checkpoint.read(str(p))
+    assert ck["version"] == 1 and list(ck["env"]["plane"]["normal"]) == [0.0, 0.0, 1.0]
+    assert np.array_equal(ck["fields"]["Velocity"], fields["Velocity"])
 
 
 def test_reader_rejects_truncation(tmp_path):

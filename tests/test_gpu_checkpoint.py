@@ -20,8 +20,10 @@ def _hooked():
         wl.clamps.append((r, st["Position"][:, 0].copy(), st["Director"][:, 0].copy()))
         wl.forces.append((r, np.array([0.2, -0.1, 0.3]), 1 + r % 2, 0.0, wl.dt * 120))
     wl.gravity = (0.0, 0.0, -9.81)
-    zmin = min(st["Position"][2].min() for st in wl.states)
-    wl.contact = (1e3, 10.0, 0.3, 10.0, zmin + 0.01, 0.0)
+    # a slightly tilted plane just above the lowest node
+    n = np.array([0.1, 0.0, 1.0]) / np.sqrt(1.01)
+    hmin = min((n @ st["Position"]).min() for st in wl.states)
+    wl.contact = (1e3, 10.0, 0.3, 10.0, hmin + 0.01, 0.0, tuple(n))
     return wl
 
 
@@ -81,6 +83,8 @@ def test_reader_returns_device_contents(tmp_path):
         for tag in STATE:
             assert np.array_equal(checkpoint.rod_field(ck, tag, r), a.get_rod(tag, r)), (r, tag)
     assert ck["env"]["gravity"] and ck["env"]["contact"] and ck["env"]["plane"]["k"] == 1e3
+    n = np.array([0.1, 0.0, 1.0]) / np.sqrt(1.01)
+    assert np.allclose(ck["env"]["plane"]["normal"], n, rtol=0, atol=1e-16) and ck["version"] == 2
     assert ck["rods"][0]["flags"] & 1 and ck["rods"][1]["ramp"] == 2 and ck["rods"][0]["dt"] == wl.dt
 
 

@@ -40,6 +40,40 @@ def test_contact_cut_matches_oracle():
     assert z.min() < 0.01 and z.min() > 0.01 - 1e-3
 
 
+def test_tilted_plane_matches_oracle():
+    """The 48-rod cut turned by a rotation, so that it rests on a tilted
+    plane with gravity along the turned axis: every path against the oracle
+    on the same turned batch, and the turned result against the turned
+    result of the level batch."""
+    from scipy.spatial.transform import Rotation
+    from tests.gpu_helpers import rotated
+    base = contact_c4(n_rods=48, n=128, steps=300)
+    R = Rotation.from_rotvec([0.3, -0.5, 0.2]).as_matrix()
+    wl = rotated(base, R)
+    sysm = oracle_system(wl)
+    sysm.step(wl.dt, wl.steps)
+    a = build_block(wl)
+    a.step(wl.dt, wl.steps)
+    b = build_block(wl)
+    b.step_fused(wl.steps, FP64)
+    c = build_block(wl)
+    c.step_fused(wl.steps, FP32_BULK)
+    e_ops, e_f64, e_f32 = rel_state_error(a, sysm), rel_state_error(b, sysm), rel_state_error(c, sysm)
+    print("op-by-op", e_ops, "fused fp64", e_f64, "fp32-bulk", e_f32)
+    assert max(e_ops.values()) <= 1e-10, e_ops
+    assert max(e_f64.values()) <= 1e-10, e_f64
+    assert e_f32["Position"] <= 1e-6, e_f32
+    level = build_block(base)
+    level.step_fused(base.steps, FP64)
+    for r in range(0, 48, 7):
+        x, y = R @ level.get_rod("Position", r), b.get_rod("Position", r)
+        assert np.abs(x - y).max() <= 1e-9, r
+    # the rods are held by the tilted plane
+    n = R @ np.array([0.0, 0.0, 1.0])
+    h = np.concatenate([n @ a.get_rod("Position", r) for r in range(48)])
+    assert h.min() < 0.01 and h.min() > 0.01 - 1e-3
+
+
 STEPS = 200
 
 

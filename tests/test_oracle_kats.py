@@ -284,3 +284,33 @@ def test_free_rods_amplify_a_last_bit_change():
                    for r in range(32))
     assert 1e-12 < rel("AngularVelocity") < 1e-7
     assert rel("Position") < 1e-12
+
+
+def test_plane_contact_is_rotation_equivariant():
+    """Turning the whole contact problem by a rotation R (rods, gravity, the
+    plane's normal) turns its trajectory by R: pins where the plane normal
+    enters the contact law (the gap, the approach speed, the slip in the
+    plane, both forces; CapsulePlane.hpp:73-80, LinearSpringDashpot.cpp:49-71).
+    Taking the slip in the x-y plane instead of the tilted plane moves the
+    rotated rods by ~1e-4 of their length here."""
+    from elasticapp_amd.workloads import contact_c4
+    from tests.gpu_helpers import oracle_system, rotated
+    wl = contact_c4(n_rods=4, n=16, steps=400)
+    # sliding rods: the friction law acts along their slip
+    for st in wl.states:
+        st["Velocity"] = st["Velocity"] + np.array([[0.3], [-0.2], [0.0]])
+    R = Rotation.from_rotvec(0.7 * np.array([1.0, 2.0, 3.0]) / np.sqrt(14.0)).as_matrix()
+    a, b = oracle_system(wl), oracle_system(rotated(wl, R))
+    a.step(wl.dt, wl.steps)
+    b.step(wl.dt, wl.steps)
+    for r in range(4):
+        xa, xb = a.rod_field(r, "Position"), b.rod_field(r, "Position")
+        assert np.abs(R @ xa - xb).max() <= 1e-12, r
+        va, vb = a.rod_field(r, "Velocity"), b.rod_field(r, "Velocity")
+        assert np.abs(R @ va - vb).max() <= 1e-9 * np.abs(va).max(), r
+        qa = a.rod_field(r, "Director").T.reshape(-1, 3, 3) @ R.T
+        assert np.abs(qa.reshape(-1, 9).T - b.rod_field(r, "Director")).max() <= 1e-12, r
+        assert np.abs(a.rod_field(r, "AngularVelocity") - b.rod_field(r, "AngularVelocity")).max() <= 1e-9, r
+    # the rods did reach the plane: some node is pressed into it
+    z = np.concatenate([a.rod_field(r, "Position")[2] for r in range(4)])
+    assert z.min() < wl.contact[5]
