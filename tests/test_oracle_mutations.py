@@ -30,6 +30,18 @@ MUTATIONS = [
     ("d1[c * E + k] = d2[c * E + k] / e[k];", "d1[c * E + k] = d2[c * E + k] * e[k];", "J w / e inverted"),
 ]
 
+# mistakes in the environment hooks, which the golden vectors do not reach;
+# the contact KATs must catch them
+ENV_MUTATIONS = [
+    ("const T vt[3] = {v[i] - vnv * nx, v[N + i] - vnv * ny, v[2 * N + i] - vnv * nz};",
+     "const T vt[3] = {v[i], v[N + i], T(0)};", "slip taken in the x-y plane"),
+    ("for (int c = 0; c < 3; ++c) fe[c * N + i] += fn * nv[c];", "fe[2 * N + i] += fn;", "normal force along z"),
+    ("std::min(T(s.plane.ct) * vtn, T(s.plane.mu) * fn)", "std::max(T(s.plane.ct) * vtn, T(s.plane.mu) * fn)",
+     "friction not capped at mu f_N"),
+    ("T(s.plane.k) * delta + T(s.plane.c) * vn", "T(s.plane.k) * delta - T(s.plane.c) * vn", "sign of the contact damping"),
+    ("fe[c * N + i] += T(s.g[c]) * m[i];", "fe[c * N + i] += T(s.g[c]);", "gravity without the mass"),
+]
+
 FLAGS = ["-O1", "-ffp-contract=off", "-std=c++20", "-fPIC", "-pthread", "-shared"]
 
 
@@ -63,10 +75,22 @@ def _failing_golden_cases():
     return fails
 
 
+def _failing_env_kats():
+    from tests import test_oracle_kats as kats
+    fails = 0
+    for t in (kats.test_free_fall_is_exact_under_position_verlet, kats.test_resting_contact_carries_the_weight,
+              kats.test_sliding_friction_decelerates_at_mu_g, kats.test_plane_contact_is_rotation_equivariant):
+        try:
+            t()
+        except AssertionError:
+            fails += 1
+    return fails
+
+
 @pytest.fixture(scope="module")
 def builds(tmp_path_factory):
     tmp = str(tmp_path_factory.mktemp("oracle_mut"))
-    jobs = [None] + MUTATIONS
+    jobs = [None] + MUTATIONS + ENV_MUTATIONS
     with concurrent.futures.ThreadPoolExecutor(max_workers=4) as ex:
         libs = list(ex.map(lambda im: _build(tmp, im[0], im[1]), enumerate(jobs)))
     yield libs
@@ -77,6 +101,7 @@ def test_unchanged_copy_passes(builds):
     pyoracle.use_library(builds[0])
     try:
         assert _failing_golden_cases() == 0
+        assert _failing_env_kats() == 0
     finally:
         pyoracle.use_library(None)
 
@@ -86,5 +111,14 @@ def test_planted_mistake_is_caught(builds, i):
     pyoracle.use_library(builds[i + 1])
     try:
         assert _failing_golden_cases() > 0, MUTATIONS[i][2]
+    finally:
+        pyoracle.use_library(None)
+
+
+@pytest.mark.parametrize("i", range(len(ENV_MUTATIONS)), ids=[m[2] for m in ENV_MUTATIONS])
+def test_planted_environment_mistake_is_caught(builds, i):
+    pyoracle.use_library(builds[1 + len(MUTATIONS) + i])
+    try:
+        assert _failing_env_kats() > 0, ENV_MUTATIONS[i][2]
     finally:
         pyoracle.use_library(None)
