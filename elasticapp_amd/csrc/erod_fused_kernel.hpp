@@ -430,35 +430,55 @@ namespace erod {
       }
     }
 
-    // external load on a node: gravity and per-node plane contact
+    // external load on a node: gravity and per-node plane contact.  kLevel:
+    // the plane z = z0 (normal (0, 0, 1)), where the general form reduces
+    // exactly, up to the sign of zeros, to the terms kept here
+    template <bool kLevel>
     __device__ __forceinline__ void node_environment(const Env& env, double m, const double x[3], const double v[3], double fe[3]) {
       if (env.gravity)
         for (int c = 0; c < 3; ++c) fe[c] += env.g[c] * m;
-      if (env.contact) {
-        const double gap = ((env.n[0] * x[0] + env.n[1] * x[1]) + env.n[2] * x[2]) - env.z0 - env.offset;
-        if (gap < 0.0) {
-          const double delta = -gap;
-          const double vnv = (env.n[0] * v[0] + env.n[1] * v[1]) + env.n[2] * v[2];
-          const double vn = -vnv;
-          const double fn = fmax(0.0, env.k * delta + env.c * vn);
-          double vt[3];
+      if (!env.contact) return;
+      double xn, vnv;  // height and velocity along the normal
+      if constexpr (kLevel) {
+        xn = x[2];
+        vnv = v[2];
+      } else {
+        xn = (env.n[0] * x[0] + env.n[1] * x[1]) + env.n[2] * x[2];
+        vnv = (env.n[0] * v[0] + env.n[1] * v[1]) + env.n[2] * v[2];
+      }
+      const double gap = xn - env.z0 - env.offset;
+      if (gap < 0.0) {
+        const double delta = -gap;
+        const double vn = -vnv;
+        const double fn = fmax(0.0, env.k * delta + env.c * vn);
+        double vt[3], vtn;  // slip in the plane
+        if constexpr (kLevel) {
+          vt[0] = v[0];
+          vt[1] = v[1];
+          vt[2] = 0.0;
+          vtn = sqrt(vt[0] * vt[0] + vt[1] * vt[1]);
+          fe[2] += fn;
+        } else {
 #pragma unroll
           for (int c = 0; c < 3; ++c) vt[c] = v[c] - vnv * env.n[c];
-          const double vtn = sqrt((vt[0] * vt[0] + vt[1] * vt[1]) + vt[2] * vt[2]);
+          vtn = sqrt((vt[0] * vt[0] + vt[1] * vt[1]) + vt[2] * vt[2]);
 #pragma unroll
           for (int c = 0; c < 3; ++c) fe[c] += fn * env.n[c];
-          if (vtn > 0.0) {
-            const double ft = fmin(env.ct * vtn, env.mu * fn);
+        }
+        if (vtn > 0.0) {
+          const double ft = fmin(env.ct * vtn, env.mu * fn);
+          const double s = ft / vtn;  // f_T along the unit slip: one division
 #pragma unroll
-            for (int c = 0; c < 3; ++c) fe[c] -= ft * (vt[c] / vtn);
-          }
+          for (int c = 0; c < (kLevel ? 2 : 3); ++c) fe[c] -= s * vt[c];
         }
       }
     }
 
     // ---- the kernel ------------------------------------------------------------------
 
-    template <typename R, int P, int NT, bool kEnv>
+    // kEnv: 0 no environment, 1 gravity and / or contact with a level plane,
+    // 2 contact with a plane of any normal
+    template <typename R, int P, int NT, int kEnv>
     __global__ void __launch_bounds__(NT) k_fused(double* __restrict__ X, double* __restrict__ Vel, double* __restrict__ Qd,
                                                   double* __restrict__ W, Consts<R> C, const RodInfo* __restrict__ rods,
                                                   const Tile* __restrict__ tiles, const uint32_t* __restrict__ tile_rods,
@@ -507,7 +527,7 @@ namespace erod {
       // is kept in registers for the launch (the ramp's times are read only
       // while a ramp is set), instead of two dependent rod-record loads per
       // step; the other instantiations have no registers to spare for it
-      constexpr bool kTailForce = kCache && !kEnv;
+      constexpr bool kTailForce = kCache && kEnv == 0;
       double tforce[3] = {0, 0, 0};
       int tramp = EROD_RAMP_NONE;
       // the tile's rod table (first lane, rod) in LDS, in the exchange-1
@@ -833,7 +853,7 @@ namespace erod {
           }
           // external loads of node k (forcing, then environment)
           double fe[3] = {0.0, 0.0, 0.0};
-          if constexpr (kEnv) node_environment(env, static_cast<double>(mk), x[j], v[j], fe);
+          if constexpr (kEnv != 0) node_environment<kEnv == 1>(env, static_cast<double>(mk), x[j], v[j], fe);
           const double dtt = dt[j];
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
@@ -865,7 +885,7 @@ namespace erod {
                 for (int c = 0; c < 3; ++c) fen[c] = ri.force[c] * rv;
               }
             }
-            if constexpr (kEnv) node_environment(env, static_cast<double>(tail_m), xt, vt, fen);
+            if constexpr (kEnv != 0) node_environment<kEnv == 1>(env, static_cast<double>(tail_m), xt, vt, fen);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
               const R fn = -md.c[c] - mgn * static_cast<R>(vt[c]);
@@ -943,23 +963,26 @@ namespace erod {
       }
     }
 
+    template <typename R, int P, int NT, int kEnv>
+    inline void launch_env(erod_batch* b, const Consts<R>& C, int steps) {
+      uint32_t* budgets = b->budgets ? b->d_steps_left : nullptr;
+      hipLaunchKernelGGL((k_fused<R, P, NT, kEnv>), dim3(static_cast<unsigned>(b->plan.n_tiles)), dim3(NT), 0, b->stream,
+                         b->d_field[EROD_POSITION], b->d_field[EROD_VELOCITY], b->d_field[EROD_DIRECTOR],
+                         b->d_field[EROD_ANGULAR_VELOCITY], C, b->d_rods, b->plan.d_tiles, b->plan.d_tile_rods,
+                         b->plan.d_tile_lane0, b->d_time, budgets, b->env, steps, b->E);
+    }
+
     // batches without gravity or contact run a kernel without environment code
-    // (and without the registers its parameters would hold)
+    // (and without the registers its parameters would hold); a level contact
+    // plane runs one without the arithmetic of a general normal
     template <typename R, int P, int NT>
     inline int launch_tiles(erod_batch* b, const Consts<R>& C, int steps) {
       if (b->plan.n_tiles == 0) return EROD_OK;
-      const dim3 grid(static_cast<unsigned>(b->plan.n_tiles)), block(NT);
-      uint32_t* budgets = b->budgets ? b->d_steps_left : nullptr;
-      if (b->env.gravity || b->env.contact)
-        hipLaunchKernelGGL((k_fused<R, P, NT, true>), grid, block, 0, b->stream, b->d_field[EROD_POSITION],
-                           b->d_field[EROD_VELOCITY], b->d_field[EROD_DIRECTOR], b->d_field[EROD_ANGULAR_VELOCITY], C,
-                           b->d_rods, b->plan.d_tiles, b->plan.d_tile_rods, b->plan.d_tile_lane0, b->d_time, budgets,
-                           b->env, steps, b->E);
-      else
-        hipLaunchKernelGGL((k_fused<R, P, NT, false>), grid, block, 0, b->stream, b->d_field[EROD_POSITION],
-                           b->d_field[EROD_VELOCITY], b->d_field[EROD_DIRECTOR], b->d_field[EROD_ANGULAR_VELOCITY], C,
-                           b->d_rods, b->plan.d_tiles, b->plan.d_tile_rods, b->plan.d_tile_lane0, b->d_time, budgets,
-                           b->env, steps, b->E);
+      const Env& e = b->env;
+      const bool level = e.n[0] == 0.0 && e.n[1] == 0.0 && e.n[2] == 1.0;
+      if (e.contact && !level) launch_env<R, P, NT, 2>(b, C, steps);
+      else if (e.gravity || e.contact) launch_env<R, P, NT, 1>(b, C, steps);
+      else launch_env<R, P, NT, 0>(b, C, steps);
       return hip_check(hipGetLastError(), "k_fused");
     }
 

@@ -368,7 +368,8 @@ namespace erod {
             for (int c = 0; c < 3; ++c) fe[c * N + i] += fn * env.n[c];
             if (vtn > 0.0) {
               const double ft = fmin(env.ct * vtn, env.mu * fn);
-              for (int c = 0; c < 3; ++c) fe[c * N + i] -= ft * (vt[c] / vtn);
+              const double s = ft / vtn;  // f_T along the unit slip
+              for (int c = 0; c < 3; ++c) fe[c * N + i] -= s * vt[c];
             }
           }
         }

@@ -639,7 +639,8 @@ namespace oracle {
           for (int c = 0; c < 3; ++c) fe[c * N + i] += fn * nv[c];
           if (vtn > T(0)) {
             const T ft = std::min(T(s.plane.ct) * vtn, T(s.plane.mu) * fn);
-            for (int c = 0; c < 3; ++c) fe[c * N + i] -= ft * (vt[c] / vtn);
+            const T sc = ft / vtn;  // f_T along the unit slip
+            for (int c = 0; c < 3; ++c) fe[c * N + i] -= sc * vt[c];
           }
         }
       }
