@@ -212,7 +212,7 @@ int erod_server_create(erod_batch* library, const uint64_t* class_capacity, cons
     }
     std::reverse(s->free_slots[c].begin(), s->free_slots[c].end());
   }
-  // every slot starts empty, with a zero budget; the pool sees the library's environment
+  // every slot starts empty, with a zero budget; each tick takes the library's environment
   if ((rc = erod_set_rod_step_budget(p, 0, 0))) {
     erod_batch_destroy(p);
     return rc;
@@ -331,7 +331,9 @@ int erod_server_tick(erod_server* s, uint64_t* done_ids, uint64_t capacity, uint
       s->active_elements += rq->elements;
     }
   }
-  // 2. one launch over every slot with budget left
+  // 2. one launch over every slot with budget left, in the library's current
+  // environment (gravity, contact), as the same rods would step there
+  p->env = s->library->env;
   if (!s->running.empty()) {
     uint64_t work = 0;
     for (erod_server::Request* rq : s->running)

@@ -217,7 +217,8 @@ typedef struct erod_server erod_server;
 int erod_server_create(erod_batch* library, const uint64_t* class_capacity, const uint64_t* class_slots, int n_classes,
                        int precision, int steps_per_tick, double hbm_budget_bytes, erod_server** out);
 int erod_server_destroy(erod_server* s);
-/* The pool batch (for its tile configuration or environment). */
+/* The pool batch (for its tile configuration).  The pool steps in the
+ * library's environment (gravity, contact), taken anew at every tick. */
 int erod_server_pool(erod_server* s, erod_batch** pool);
 /* Queue request `id`: the given library rods, advanced `steps` steps each with
  * their own dt.  Rejected if it could not fit the empty pool. */

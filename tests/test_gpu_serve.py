@@ -151,3 +151,27 @@ def test_server_rejects_what_cannot_fit():
     with pytest.raises(ErodError):
         Server(lib, [(1024, 100000)], precision=FP32_BULK, hbm_budget_bytes=1e9)   # over budget
     srv.close()
+
+
+def test_server_steps_in_the_library_environment():
+    """Gravity set on the library after the server exists reaches the pool at
+    the next tick: a served request equals its rods stepped directly in a
+    batch with the same gravity."""
+    wl, lib = _library(n_rods=6, seed=43)
+    dts = [float(wl.dt * (1.0 + 0.01 * r)) for r in range(len(wl.n_elems))]
+    srv = Server(lib, [(128, 8)], precision=FP64, steps_per_tick=50, hbm_budget_bytes=1e9)
+    lib.set_gravity([0.0, 0.0, -9.81])
+    rods, steps = [0, 2, 5], 120
+    srv.submit(0, rods, steps)
+    got = []
+    while not got:
+        got = srv.tick()
+    sub = Workload("sub", [wl.n_elems[r] for r in rods], [wl.states[r] for r in rods], wl.dt, steps,
+                   forces=[(i,) + tuple(wl.forces[r][1:]) for i, r in enumerate(rods)], gravity=(0.0, 0.0, -9.81))
+    ref = build_block(sub)
+    ref.set_rod_dt(np.array([dts[r] for r in rods]))
+    ref.step_fused(steps, FP64)
+    for k in range(len(rods)):
+        for t in STATE:
+            assert np.array_equal(srv.read(0, k, t), ref.get_rod(t, k)), (k, t)
+    srv.close()
