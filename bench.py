@@ -69,6 +69,31 @@ def measured_traffic(lanes, epl, spl):
     return None
 
 
+# VALU issue peak of MI355X: 256 CUs x 2.4 GHz, each CU issuing one wave64
+# VALU instruction per cycle (4 SIMDs of 16 lanes, 4 cycles per wave64
+# instruction), which is the 78.6 TFLOP/s FP64 vector peak counted in
+# instructions (64 lanes x 2 flops per FMA)
+VALU_PEAK_GINSTR = 256 * 2.4
+
+
+def measured_valu(lanes, epl, spl):
+    """(waves per launch, VALU instructions per wave and step) of the
+    FP32-bulk kernel at this tile shape on the ragged batch, from the
+    rocprofv3 SQ counter summary (`valu lanes=L epl=P spl=S waves=W
+    insts_per_wave_step=I`), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                w = line.strip().strip("`").split()
+                if len(w) == 6 and w[0] == "valu":
+                    kv = dict(x.split("=", 1) for x in w[1:])
+                    if (int(kv["lanes"]), int(kv["epl"]), int(kv["spl"])) == (lanes, epl, spl):
+                        return int(kv["waves"]), float(kv["insts_per_wave_step"])
+    except (OSError, ValueError, KeyError):
+        pass
+    return None
+
+
 def stream_copy_gbs(torch, nbytes=2 << 30, reps=10):
     """Device-to-device copy rate (read + write bytes / s) of a 2 GiB buffer:
     the measured HBM ceiling beside the 8 TB/s peak."""
@@ -447,6 +472,16 @@ def main():
     blk32.close()
     traffic = measured_traffic(lanes, epl, spl)
     traffic_s = measured_traffic(lanes, epl, 1)
+    valu = measured_valu(lanes, epl, spl)
+    roofline_valu = None
+    if valu is not None:
+        waves, ipws = valu
+        g = waves * ipws * steps_per_launch / (kms * 1e-3) / 1e9
+        roofline_valu = {"bound": "valu", "achieved": g, "peak": VALU_PEAK_GINSTR, "unit": "G wave64 VALU instr/s",
+                         "frac": g / VALU_PEAK_GINSTR, "insts_per_wave_step": ipws, "waves_per_launch": waves,
+                         "note": "what bounds the kernel once its state is on chip: the VALU instruction count per "
+                                 "wave and step (rocprofv3 SQ_INSTS_VALU, committed summary) over the event-timed "
+                                 "launch, against one wave64 VALU instruction per CU per cycle"}
 
     # ---- plain FP64 (one kernel per reference operator) ----------------------------------
     blkp = make_block(wl)
@@ -499,6 +534,7 @@ def main():
                                "kernel_ms_per_launch": kms_s, "launches": launches_s, "steps_per_launch": 1,
                                "algorithmic_bytes_per_launch": bytes_s,
                                "element_steps_per_s": elems / (kms_s * 1e-3)},
+        "roofline_valu": roofline_valu,
         "stream_copy_gbs": copy_gbs,
         "c0_c1": small,
         "c4_contact": c4,
