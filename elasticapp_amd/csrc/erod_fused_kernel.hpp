@@ -837,19 +837,20 @@ namespace erod {
           // the Voronoi terms (P, Rv) are zero at the last one, and x - 0, 0 - x
           // and 0 + x are exact, as with the reference's zero ghosts.
           R f[3];
+          // (madd is one FMA for FP32-bulk and, for FP64, a product and a sum
+          // rounded on their own: the values of the reference's expressions)
           R vk[3] = {static_cast<R>(v[j][0]), static_cast<R>(v[j][1]), static_cast<R>(v[j][2])};
           const R mg = mk * gf;
 #pragma unroll
-          for (int c = 0; c < 3; ++c) f[c] = (md.c[c] - cm[c]) - mg * vk[c];
+          for (int c = 0; c < 3; ++c) f[c] = madd(-mg, vk[c], md.c[c] - cm[c]);
           // internal torque on element k
           const R em = R(0.5) * (mk + mkn);
           const R eg = em * gt;
           R tau[3];
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            const R d = md.P[c] - Pm[c];
-            const R qd = R(0.5) * (md.Rv[c] + Rm[c]);
-            tau[c] = ((((d + qd) + md.shear[c]) + md.lag[c]) + md.unst[c]) - eg * static_cast<R>(w[j][c]);
+            const R dq = madd(R(0.5), md.Rv[c] + Rm[c], md.P[c] - Pm[c]);
+            tau[c] = madd(-eg, static_cast<R>(w[j][c]), ((dq + md.shear[c]) + md.lag[c]) + md.unst[c]);
           }
           // external loads of node k (forcing, then environment)
           double fe[3] = {0.0, 0.0, 0.0};
@@ -857,7 +858,8 @@ namespace erod {
           const double dtt = dt[j];
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            const R a = (f[c] + static_cast<R>(fe[c])) * im;
+            // (without an environment the external load is zero)
+            const R a = (kEnv != 0 ? f[c] + static_cast<R>(fe[c]) : f[c]) * im;
             v[j][c] = maddd<R>(dtt, static_cast<double>(a), v[j][c]);
           }
 #pragma unroll
@@ -888,7 +890,7 @@ namespace erod {
             if constexpr (kEnv != 0) node_environment<kEnv == 1>(env, static_cast<double>(tail_m), xt, vt, fen);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-              const R fn = -md.c[c] - mgn * static_cast<R>(vt[c]);
+              const R fn = madd(-mgn, static_cast<R>(vt[c]), -md.c[c]);
               const R a = (fn + static_cast<R>(fen[c])) * tail_im;
               vt[c] = maddd<R>(dtt, static_cast<double>(a), vt[c]);
             }
