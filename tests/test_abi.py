@@ -68,3 +68,28 @@ def test_no_cpu_fallback_without_gpu(lib):
         pytest.skip("a GPU is visible")
     with pytest.raises(ea.ErodError):
         ea.DeviceBlock([4, 5])
+
+
+def test_bad_arguments_fail_before_any_device_call(lib):
+    """Arguments the library can refuse on the host are refused with a status
+    and a message, without touching a device (so this runs with no GPU)."""
+    import ctypes
+    import elasticapp_amd as ea
+    L = ea.load_library()
+    out = ctypes.c_void_p()
+    one = (ctypes.c_uint64 * 1)(1)
+    assert L.erod_batch_create(one, 1, 1, ctypes.byref(out)) != 0 and not out.value
+    assert b"< 2 elements" in L.erod_last_error()
+    two = (ctypes.c_uint64 * 1)(8)
+    assert L.erod_batch_create(two, 0, 1, ctypes.byref(out)) != 0
+    # a rod count beyond 32-bit rod ids is refused before the array is read
+    assert L.erod_batch_create(two, 1 << 32, 1, ctypes.byref(out)) != 0 and b"2^32" in L.erod_last_error()
+    assert L.erod_batch_destroy(None) == 0
+    t = ctypes.c_double(0.0)
+    assert L.erod_step(None, ctypes.byref(t), 1e-3, 1) != 0
+    assert L.erod_step(None, None, 1e-3, 1) != 0 and b"null time" in L.erod_last_error()
+    n = (ctypes.c_double * 3)(0.0, 0.0, 1.0)
+    assert L.erod_set_plane_normal(None, n) != 0
+    done = ctypes.c_uint64()
+    assert L.erod_server_tick(None, None, 0, ctypes.byref(done)) != 0 and b"null server" in L.erod_last_error()
+    assert L.erod_load(b"/nonexistent/erod.ckpt", ctypes.byref(out)) != 0 and b"cannot open" in L.erod_last_error()
