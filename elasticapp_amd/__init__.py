@@ -461,6 +461,10 @@ class _PoolView(DeviceBlock):
         self._h = handle
         self.n_elems = list(n_elems)
         self.damped = damped
+        self.time = 0.0
+        e = ctypes.c_uint64()
+        _check(_LIB.erod_batch_slots(self._h, ctypes.byref(e)))
+        self.element_slots = e.value
         self.starts = []
         for r in range(len(self.n_elems)):
             s_ = ctypes.c_uint64()
@@ -499,6 +503,9 @@ class Server:
 
     def close(self):
         if getattr(self, "_h", None):
+            # the pool view goes first: its handle dies with the server
+            if getattr(self, "pool", None) is not None:
+                self.pool.close()
             _LIB.erod_server_destroy(self._h)
             self._h = None
 

@@ -175,3 +175,18 @@ def test_server_steps_in_the_library_environment():
         for t in STATE:
             assert np.array_equal(srv.read(0, k, t), ref.get_rod(t, k)), (k, t)
     srv.close()
+
+
+def test_pool_view_reads_fields_and_dies_with_the_server():
+    """The server's pool reads as a DeviceBlock (a whole field included) while
+    the server lives, and refuses calls once it is closed."""
+    wl, lib = _library(n_rods=4, seed=44)
+    srv = Server(lib, [(128, 4)], precision=FP64, steps_per_tick=20, hbm_budget_bytes=1e9)
+    srv.submit(0, [1], 40)
+    while not srv.tick():
+        pass
+    x = srv.pool.get_field("Position")
+    assert x.shape == (3, srv.pool.slots(0)) and np.isfinite(x).all()
+    srv.close()
+    with pytest.raises(ErodError):
+        srv.pool.get_field("Position")
