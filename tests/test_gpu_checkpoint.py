@@ -93,3 +93,30 @@ def test_load_rejects_a_foreign_file(tmp_path):
     p.write_bytes(b"ERODCKP1" + b"\x00" * 40)
     with pytest.raises((ErodError, ValueError)):
         DeviceBlock.load(str(p))
+
+
+def test_version_one_file_loads_with_the_level_plane(tmp_path):
+    """A version 1 checkpoint (written before the plane normal existed: the
+    same layout without the normal's 24 bytes) loads and steps as the batch
+    it came from with its plane set level."""
+    import struct
+    wl = _hooked()
+    wl.contact = tuple(wl.contact[:6])  # level plane
+    a = build_block(wl)
+    a.step_fused(50, FP64)
+    path = tmp_path / "v2.ckpt"
+    a.save(str(path))
+    data = bytearray(path.read_bytes())
+    n = len(wl.n_elems)
+    # header 32 B; n_elems, n_active (u64), steps_left (u32); budgets (u8);
+    # rod clocks (f64); 168 B per rod record; environment 80 B; then the normal
+    at = 32 + n * (8 + 8 + 4) + 1 + 8 * n + 168 * n + 80
+    assert struct.unpack_from("<3d", data, at) == (0.0, 0.0, 1.0)
+    v1 = data[:8] + struct.pack("<I", 1) + data[12:at] + data[at + 24:]
+    p1 = tmp_path / "v1.ckpt"
+    p1.write_bytes(bytes(v1))
+    assert checkpoint.read(str(p1))["version"] == 1
+    b = DeviceBlock.load(str(p1))
+    a.step_fused(50, FP64)
+    b.step_fused(50, FP64)
+    _same(a, b, STATE)
