@@ -342,7 +342,7 @@ def c3_leg(args, torch, capacity):
     out = {"workload": "%d requests, U{1..64} rods of %d templates (n logU[32,512], materials as the ragged batch, "
                        "own stable dt), horizons logU[1e3,1e5] steps, Poisson arrivals" % (len(reqs), len(n_lib)),
            "lambda_per_s": lam, "target_utilization": 0.7, "server_capacity_element_steps_per_s": server_capacity,
-           "server_capacity_ticks": k1 - k0, "capacity_element_steps_per_s": capacity,
+           "server_capacity_ticks": k1 - k0, "ragged_batch_element_steps_per_s": capacity,
            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
            "mean_s": float(np.mean(lat)), "max_s": float(np.max(lat)), "throughput": thr, "unit": "element-steps/s",
            "non_finite_results": non_finite,
