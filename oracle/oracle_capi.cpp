@@ -195,6 +195,11 @@ int oracle_set_plane_normal(void* h, const double* n3) {
   return 0;
 }
 
+// the ramp of an end force at time t (0 none, 1 linear, 2 sin)
+double oracle_ramp_value(int kind, double t, double start, double interval) {
+  return oracle::ramp_value(kind, t, start, interval);
+}
+
 double oracle_time(void* h) { return static_cast<Sys*>(h)->time; }
 void oracle_set_time(void* h, double t) { static_cast<Sys*>(h)->time = t; }
 

@@ -62,6 +62,8 @@ def _lib(path=None):
         L.oracle_set_gravity.argtypes = [vp, dp]
         L.oracle_set_plane_contact.argtypes = [vp] + [ctypes.c_double] * 6
         L.oracle_set_plane_normal.argtypes = [vp, dp]
+        L.oracle_ramp_value.restype = ctypes.c_double
+        L.oracle_ramp_value.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.oracle_time.restype = ctypes.c_double
         L.oracle_time.argtypes = [vp]
         L.oracle_set_time.argtypes = [vp, ctypes.c_double]
@@ -229,3 +231,8 @@ def so3_add_assign(Q, u):
     u = np.ascontiguousarray(u, dtype=np.float64)
     _lib().oracle_so3_add_assign(_dptr(Q), Q.shape[2], _dptr(u))
     return Q
+
+
+def ramp_value(kind, t, start, interval):
+    """The ramp factor of an end force at time t (0 none, 1 linear, 2 sin)."""
+    return _lib().oracle_ramp_value(int(kind), float(t), float(start), float(interval))

@@ -378,3 +378,24 @@ def test_finite_flags():
     blk.set_rod("Velocity", 3, v)
     flags = blk.check_finite()
     assert flags.tolist() == [0, 0, 0, 1, 0]
+
+
+@pytest.mark.parametrize("ramp", [1, 2])
+def test_device_ramp_matches_the_reference_known_answers(ramp):
+    """The end force the device hook applies at time t, F ramp(t), at the
+    points Test_Rampup.cpp checks (start -2, interval 4) and between them,
+    against the oracle's ramp (the device sin may differ in the last bit)."""
+    from oracle import pyoracle
+    st = straight_rod_state(6, 0.01, 1.0, [0, 0, 0], [1, 0, 0], [0, 1, 0], 1e3, 1e6, 4e5, 0.1, 0.1)
+    blk = DeviceBlock([6])
+    for tag, a in st.items():
+        blk.set_rod(tag, 0, a)
+    s, T = -2.0, 4.0
+    blk.add_end_force(0, [1.0, -2.0, 0.5], ramp=ramp, start=s, interval=T)
+    for t in [-2.5, s] + [s + f * T for f in np.linspace(0.05, 1.0, 20)] + [s + 1.5 * T]:
+        blk.compute_internal_dynamics(t)
+        blk.apply_pre_dynamic(t)
+        got = blk.get_rod("ExternalLoads", 0)[:, -1]
+        want = pyoracle.ramp_value(ramp, t, s, T)
+        assert np.abs(got - want * np.array([1.0, -2.0, 0.5])).max() <= 2e-16 * 2.0, (t, got, want)
+    blk.close()

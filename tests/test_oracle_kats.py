@@ -314,3 +314,39 @@ def test_plane_contact_is_rotation_equivariant():
     # the rods did reach the plane: some node is pressed into it
     z = np.concatenate([a.rod_field(r, "Position")[2] for r in range(4)])
     assert z.min() < wl.contact[5]
+
+
+# --- ramps (Rampup.hpp:95-150, 165-222; Test_Rampup.cpp) -----------------------------
+
+def test_ramps_match_the_reference_known_answers():
+    """The values Test_Rampup.cpp checks for make_linear_rampup and
+    make_sin_rampup, canonical (start 0, interval 1) and shifted (start -2,
+    interval 4): 0 before the start, the profile inside, 1 after."""
+    r = pyoracle.ramp_value
+    for v in (-0.2, 0.0, 0.2, 0.5, 0.8, 1.0, 1.2):
+        assert r(1, v, 0.0, 1.0) == min(max(v, 0.0), 1.0), v      # exact, as the reference checks
+    s, T = -2.0, 4.0
+    assert r(1, -2.5, s, T) == 0.0 and r(1, s, s, T) == 0.0
+    for f in (0.3, 0.5, 0.7):
+        assert abs(r(1, s + f * T, s, T) - f) <= 1e-15
+    assert r(1, s + T, s, T) == 1.0 and r(1, s + 1.5 * T, s, T) == 1.0
+    for s, T in ((0.0, 1.0), (-2.0, 4.0)):
+        assert r(2, s - 0.5 * T / 4, s, T) == 0.0 and abs(r(2, s, s, T)) <= 1e-16
+        assert abs(r(2, s + 0.5 * T, s, T) - 0.5) <= 1e-15
+        for f in (0.25, 0.3):
+            assert abs(r(2, s + f * T, s, T) + r(2, s + (1 - f) * T, s, T) - 1.0) <= 1e-15
+        assert abs(r(2, s + T, s, T) - 1.0) <= 1e-15 and r(2, s + 1.5 * T, s, T) == 1.0
+    # no ramp: the force from the start
+    assert r(0, -1.0, 0.0, 1.0) == 1.0
+
+
+def test_sin_ramp_is_the_smooth_step():
+    """Inside its interval the sin ramp is tau - sin(2 pi tau) / (2 pi) of the
+    elapsed fraction tau, the integral of 1 - cos(2 pi tau): zero slope at both
+    ends (checked against scipy's quadrature of that slope)."""
+    from scipy.integrate import quad
+    s, T = 0.7, 2.5
+    for tau in np.linspace(0.0, 1.0, 41):
+        got = pyoracle.ramp_value(2, s + tau * T, s, T)
+        want = quad(lambda u: 1.0 - np.cos(2.0 * np.pi * u), 0.0, tau, epsabs=1e-15)[0]
+        assert abs(got - want) <= 1e-14, tau
