@@ -81,13 +81,13 @@ namespace erod {
   int fused_shape_lanes(int si) { return fused::kShapes[si].P * fused::kShapes[si].NT; }
   int fused_shape_epl(int si) { return fused::kShapes[si].P; }
 
-  // Feasible shapes: capacity P * NT holds the longest rod, and every rod has
-  // at least P elements (a thread then holds at most one rod end).
   // rods the fused kernels advance: non-empty slots with steps left
   static bool planned(const erod_batch* b, uint64_t r) {
     return b->n_active[r] != 0 && (!b->budgets || b->steps_left[r] != 0);
   }
 
+  // Feasible shapes: capacity P * NT holds the longest rod, and every rod has
+  // at least P elements (a thread then holds at most one rod end).
   static bool shape_feasible(const erod_batch* b, int si) {
     uint64_t maxn = 0, minn = ~uint64_t(0);
     for (uint64_t r = 0; r < b->n_rods; ++r) {
