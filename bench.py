@@ -151,13 +151,13 @@ def cpu_baseline(wl, budget_s=12.0):
                       "%.1f s, %s" % (n_rods, elems, steps, el, cpu_model)}
 
 
-def c4_leg(args, torch):
+def contact_leg(args, torch):
     """Rods on a plane under gravity with friction: 100,000 rods x 128
     elements, FP32-bulk fused steps in hipGraph chunks, beside a few plain
     FP64 steps."""
     from elasticapp_amd import FP32_BULK
-    from elasticapp_amd.workloads import contact_c4
-    wl = contact_c4(steps=args.c4_steps)
+    from elasticapp_amd.workloads import rods_on_plane
+    wl = rods_on_plane(steps=args.contact_steps)
     n = np.array(wl.n_elems)
     elems = int(n.sum())
     blk = make_block(wl)
@@ -170,36 +170,36 @@ def c4_leg(args, torch):
     blk.autotune(FP32_BULK, args.chunk)
     blk.step_fused_graph(args.chunk, args.chunk, FP32_BULK)
     sync()
-    el = timed(lambda: blk.step_fused_graph(args.c4_steps, args.chunk, FP32_BULK), sync)
+    el = timed(lambda: blk.step_fused_graph(args.contact_steps, args.chunk, FP32_BULK), sync)
     kms, launches = blk.last_kernel_ms()
     lanes, epl, spl = blk.fused_config()
     flags = blk.check_finite()
     assert not flags.any(), "non-finite state in %d contact rods" % int(flags.sum())
     z = blk.get_field("Position")[2]
     zmin = float(min(z[s0:s0 + k + 1].min() for s0, k in zip(blk.starts, n)))  # rod nodes, not ghosts
-    v = elems * args.c4_steps / el
-    bytes_launch = float(b32(n).sum()) * (args.c4_steps // launches)
+    v = elems * args.contact_steps / el
+    bytes_launch = float(b32(n).sum()) * (args.contact_steps // launches)
     achieved = bytes_launch / (kms * 1e-3) / 1e9
-    log("c4 contact: %d rods x %d, %.3f s for %d steps -> %.3e element-steps/s, %.0f GB/s algorithmic"
-        % (len(n), n[0], el, args.c4_steps, v, achieved))
+    log("rods on a plane: %d rods x %d, %.3f s for %d steps -> %.3e element-steps/s, %.0f GB/s algorithmic"
+        % (len(n), n[0], el, args.contact_steps, v, achieved))
     # plain FP64 steps continue from the fused state on the same batch
     blk.step(wl.dt, 2)
     sync()
-    elp = timed(lambda: blk.step(wl.dt, args.c4_plain_steps), sync)
+    elp = timed(lambda: blk.step(wl.dt, args.contact_plain_steps), sync)
     blk.close()
-    vp = elems * args.c4_plain_steps / elp
-    log("c4 contact plain fp64: %.3e element-steps/s" % vp)
+    vp = elems * args.contact_plain_steps / elp
+    log("rods on a plane, plain fp64: %.3e element-steps/s" % vp)
     cpu = None if args.no_cpu else _cpu_rate(wl, 1024, 5.0, min(16, len(os.sched_getaffinity(0))))
     return {"cpu_baseline": cpu,
-            "workload": "contact_c4: %d rods x %d elements, L 1 m, r 0.01 m, rho 1000, E 1e6, rest curvature "
+            "workload": "rods_on_plane: %d rods x %d elements, L 1 m, r 0.01 m, rho 1000, E 1e6, rest curvature "
                         "N(0,0.5), gravity -9.81 z, plane z=0 k 1e3 c 10 mu 0.3 ct 10" % (len(n), n[0]),
-            "elements": elems, "dt": wl.dt, "steps": args.c4_steps, "value": v, "unit": "element-steps/s",
-            "ms_per_step": el * 1e3 / args.c4_steps, "tile_lanes": lanes, "elements_per_thread": epl,
+            "elements": elems, "dt": wl.dt, "steps": args.contact_steps, "value": v, "unit": "element-steps/s",
+            "ms_per_step": el * 1e3 / args.contact_steps, "tile_lanes": lanes, "elements_per_thread": epl,
             "steps_per_launch": spl, "min_z": zmin,
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "kernel_ms_per_launch": kms, "launches": launches,
                          "algorithmic_bytes_per_launch": bytes_launch},
-            "fp64_plain": {"value": vp, "steps": args.c4_plain_steps}, "ratio_fp32bulk_vs_plain_fp64": v / vp}
+            "fp64_plain": {"value": vp, "steps": args.contact_plain_steps}, "ratio_fp32bulk_vs_plain_fp64": v / vp}
 
 
 def _cpu_rate(wl, n_rods, budget_s, threads):
@@ -223,17 +223,17 @@ def _cpu_rate(wl, n_rods, budget_s, threads):
             "sample": "%d rods (%d elements), %d FP64 steps, %.1f s" % (n_rods, elems, steps, el)}
 
 
-def small_configs_leg(args, torch, cpu):
+def small_batches_leg(args, torch, cpu):
     """The Timoshenko cantilever (1 rod x 100, 50,000 steps) and the uniform
     batch (1024 rods x 64, 10,000 steps): FP64 fused and plain rates, the
     fused FP64 error against the oracle for the cantilever, the roofline
     fraction of the uniform batch, and the oracle timed beside them."""
     from elasticapp_amd import FP64
-    from elasticapp_amd.workloads import build_block, timoshenko_c0, uniform_c1
+    from elasticapp_amd.workloads import build_block, timoshenko_cantilever, uniform_1024x64
     from tests.gpu_helpers import oracle_system, rel_state_error
     threads = min(16, len(os.sched_getaffinity(0)))
     out = {}
-    wl = timoshenko_c0(steps=50000)
+    wl = timoshenko_cantilever(steps=50000)
     blk = build_block(wl)
     sync = lambda: (blk.synchronize(), torch.cuda.synchronize())
     blk.step_fused(1000, FP64)
@@ -246,14 +246,14 @@ def small_configs_leg(args, torch, cpu):
     err = rel_state_error(blk, sysm)
     plain = build_block(wl)
     elp = timed(lambda: plain.step(wl.dt, 1000), lambda: (plain.synchronize(), torch.cuda.synchronize()))
-    out["c0_timoshenko"] = {"steps": wl.steps, "fp64_fused": 100 * wl.steps / el, "fp64_plain": 100 * 1000 / elp,
+    out["timoshenko_cantilever"] = {"steps": wl.steps, "fp64_fused": 100 * wl.steps / el, "fp64_plain": 100 * 1000 / elp,
                             "unit": "element-steps/s", "max_rel_err_vs_oracle": max(err.values()),
                             "tip": blk.get_rod("Position", 0)[:, -1].tolist(),
                             "cpu_baseline": {"value": 100 * wl.steps / cpu_el, "unit": "element-steps/s", "cores": 1,
                                              "kind": "port", "sample": "the whole run, 1 thread"}}
-    log("c0 timoshenko: fp64 fused %.3e, plain %.3e element-steps/s, max rel err vs oracle %.1e"
-        % (out["c0_timoshenko"]["fp64_fused"], out["c0_timoshenko"]["fp64_plain"], max(err.values())))
-    wl = uniform_c1(steps=10000)
+    log("timoshenko cantilever (100 elements): fp64 fused %.3e, plain %.3e element-steps/s, max rel err vs oracle %.1e"
+        % (out["timoshenko_cantilever"]["fp64_fused"], out["timoshenko_cantilever"]["fp64_plain"], max(err.values())))
+    wl = uniform_1024x64(steps=10000)
     n = np.array(wl.n_elems)
     blk = build_block(wl)
     blk.set_fused_config(0, 0, 100)
@@ -265,16 +265,16 @@ def small_configs_leg(args, torch, cpu):
     kms, launches = blk.last_kernel_ms()
     achieved = float(b64(n).sum()) * (wl.steps // launches) / (kms * 1e-3) / 1e9
     elems = int(n.sum())
-    out["c1_uniform"] = {"steps": wl.steps, "fp64_fused": elems * wl.steps / el, "unit": "element-steps/s",
+    out["uniform_1024x64"] = {"steps": wl.steps, "fp64_fused": elems * wl.steps / el, "unit": "element-steps/s",
                          "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                       "frac": achieved / HBM_PEAK_GBS, "kernel_ms_per_launch": kms,
                                       "algorithmic_bytes_per_launch": float(b64(n).sum()) * (wl.steps // launches)},
                          "cpu_baseline": None if not cpu else _cpu_rate(wl, 1024, 5.0, threads)}
-    log("c1 uniform: fp64 fused %.3e element-steps/s, %.0f GB/s algorithmic" % (out["c1_uniform"]["fp64_fused"], achieved))
+    log("1024 x 64 uniform rods: fp64 fused %.3e element-steps/s, %.0f GB/s algorithmic" % (out["uniform_1024x64"]["fp64_fused"], achieved))
     return out
 
 
-def c3_leg(args, torch, capacity):
+def serving_leg(args, torch, capacity):
     """Continuous batching: Poisson arrivals of requests (1-64 template rods of
     32-512 elements, horizons of 1e3-1e5 steps) served from a pool of rod slots
     by the FP32-bulk fused kernel; latency = completion - arrival.  The arrival
@@ -288,13 +288,13 @@ def c3_leg(args, torch, capacity):
     lib = build_block(wl)
     lib.set_rod_dt(dts)
     n_lib = np.array(wl.n_elems)
-    reqs = serving_requests(args.c3_requests, len(n_lib))
+    reqs = serving_requests(args.serve_requests, len(n_lib))
     work = np.array([n_lib[r].sum() * h for r, h in reqs], dtype=np.float64)
     # mean work of the request distribution (sampled over many draws, not just this run's)
     mean_work = float(np.mean([n_lib[r].sum() * h for r, h in serving_requests(20000, len(n_lib), seed=99)]))
     classes = [(64, 8192), (128, 8192), (256, 8192), (512, 8192)]
     # the server's capacity: a backlog keeps the queue non-empty over the timed ticks
-    spt = args.c3_steps_per_tick
+    spt = args.serve_steps_per_tick
     sat = Server(lib, classes, precision=FP32_BULK, steps_per_tick=spt, hbm_budget_bytes=200e9)
     for i, (rods, h) in enumerate(serving_requests(3000, len(n_lib), seed=3006)):
         sat.submit(i, rods, h)
@@ -310,7 +310,7 @@ def c3_leg(args, torch, capacity):
     k1, e1 = sat.stats()
     server_capacity = (e1 - e0) / sat_wall
     sat.close()
-    log("c3 server capacity: %.3e element-steps/s over %d backlogged ticks" % (server_capacity, k1 - k0))
+    log("serving, server capacity: %.3e element-steps/s over %d backlogged ticks" % (server_capacity, k1 - k0))
     lam = 0.7 * server_capacity / mean_work
     rng = np.random.default_rng(3005)
     arrivals = np.cumsum(rng.exponential(1.0 / lam, len(reqs)))
@@ -350,7 +350,7 @@ def c3_leg(args, torch, capacity):
            "wall_s": wall,
            "ticks": ticks, "steps_per_tick": spt,
            "pool_classes": classes, "pool_bytes": pool_bytes, "hbm_budget_bytes": 200e9}
-    log("c3 serving: lambda %.1f/s, p50 %.3f s, p99 %.3f s, %.3e element-steps/s (%.0f%% of the server's capacity), "
+    log("serving: lambda %.1f/s, p50 %.3f s, p99 %.3f s, %.3e element-steps/s (%.0f%% of the server's capacity), "
         "%d ticks" % (lam, out["p50_s"], out["p99_s"], thr, 100 * thr / server_capacity, ticks))
     return out
 
@@ -363,11 +363,11 @@ def main():
     ap.add_argument("--chunk", type=int, default=100, help="steps per captured hipGraph replay")
     ap.add_argument("--plain-steps", type=int, default=200, help="steps timed on the op-by-op FP64 path")
     ap.add_argument("--stream-steps", type=int, default=200, help="steps timed at one step per launch")
-    ap.add_argument("--c4-steps", type=int, default=2000, help="steps of the contact batch (0: skip it)")
-    ap.add_argument("--c4-plain-steps", type=int, default=20)
-    ap.add_argument("--c3-requests", type=int, default=2000, help="requests of the serving leg (0: skip it)")
-    ap.add_argument("--c3-steps-per-tick", type=int, default=500, help="fused steps per server tick")
-    ap.add_argument("--small-configs", type=int, default=1, help="1: also run the cantilever and the uniform batch")
+    ap.add_argument("--contact-steps", type=int, default=2000, help="steps of the contact batch (0: skip it)")
+    ap.add_argument("--contact-plain-steps", type=int, default=20)
+    ap.add_argument("--serve-requests", type=int, default=2000, help="requests of the serving leg (0: skip it)")
+    ap.add_argument("--serve-steps-per-tick", type=int, default=500, help="fused steps per server tick")
+    ap.add_argument("--small-batches", type=int, default=1, help="1: also run the cantilever and the uniform batch")
     ap.add_argument("--tile", default="", help="LANES,EPL: fixed tile shape instead of the autotuner's choice")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rods", type=int, default=4096)
@@ -376,21 +376,21 @@ def main():
         print("bench.py runs on one GPU only: liberod has no multi-GPU path (rods are independent, so a batch "
               "would shard by rods with no exchange per step)", file=sys.stderr)
         sys.exit(2)
-    if args.steps % args.chunk or args.warmup % args.chunk or args.c4_steps % args.chunk:
-        print("--steps, --warmup and --c4-steps must be multiples of --chunk", file=sys.stderr)
+    if args.steps % args.chunk or args.warmup % args.chunk or args.contact_steps % args.chunk:
+        print("--steps, --warmup and --contact-steps must be multiples of --chunk", file=sys.stderr)
         sys.exit(2)
 
     import torch
     from elasticapp_amd import FP32_BULK, FP64
-    from elasticapp_amd.workloads import ragged_c2
+    from elasticapp_amd.workloads import ragged_4096
 
     torch.cuda.init()
     dev_name = torch.cuda.get_device_name(0)
     log("device", dev_name)
-    wl = ragged_c2(n_rods=args.rods, steps=args.steps)
+    wl = ragged_4096(n_rods=args.rods, steps=args.steps)
     n = np.array(wl.n_elems)
     elems = int(n.sum())
-    log("workload ragged_c2: %d rods, %d elements, dt per rod %.3e .. %.3e" % (
+    log("workload ragged_4096: %d rods, %d elements, dt per rod %.3e .. %.3e" % (
         len(n), elems, float(np.min(wl.info["rod_dt"])), float(np.max(wl.info["rod_dt"]))))
 
     copy_gbs = stream_copy_gbs(torch)
@@ -492,9 +492,9 @@ def main():
     log("plain fp64: %.3f s for %d steps -> %.3e element-steps/s" % (elp, args.plain_steps, vp))
     blkp.close()
 
-    small = small_configs_leg(args, torch, not args.no_cpu) if args.small_configs else None
-    c4 = c4_leg(args, torch) if args.c4_steps > 0 else None
-    c3 = c3_leg(args, torch, v32) if args.c3_requests > 0 else None
+    small = small_batches_leg(args, torch, not args.no_cpu) if args.small_batches else None
+    contact = contact_leg(args, torch) if args.contact_steps > 0 else None
+    serving = serving_leg(args, torch, v32) if args.serve_requests > 0 else None
 
     cpu = None if args.no_cpu else cpu_baseline(wl)
     if cpu:
@@ -513,7 +513,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 (f64 state)",
         "data": "synthetic",
-        "config": {"workload": "ragged_c2: 4096 rods, n log-uniform [16,1024] (seed 2002), L U[0.5,3] m, "
+        "config": {"workload": "ragged_4096: 4096 rods, n log-uniform [16,1024] (seed 2002), L U[0.5,3] m, "
                                "r U[0.005,0.05] m, E logU[1e5,1e7], rho U[500,5000], damping U[0.01,0.5], "
                                "N(0,0.5) rest curvature, N(0,1e-2) velocities, N(0,1) N tip forces; "
                                "each rod its own stable dt (axial Courant and rotational limits)",
@@ -536,9 +536,9 @@ def main():
                                "element_steps_per_s": elems / (kms_s * 1e-3)},
         "roofline_valu": roofline_valu,
         "stream_copy_gbs": copy_gbs,
-        "c0_c1": small,
-        "c4_contact": c4,
-        "c3_serving": c3,
+        "cantilever_and_uniform": small,
+        "rods_on_plane": contact,
+        "serving": serving,
         "cpu_baseline": cpu,
         "fp64_fused": {"value": v64, "ms_per_step": el64 * 1e3 / args.steps},
         "fp64_plain": {"value": vp, "steps": args.plain_steps},

@@ -5,13 +5,14 @@ Each builder returns a Workload: per-rod element counts, per-rod state dicts
 descriptors.  The same arrays feed the GPU paths and the CPU oracle, so the
 random generator does not affect parity.
 
-  timoshenko_c0      1 rod x 100 elements, clamped, tip force (0, -15, 0)
-  uniform_c1         1024 rods x 64 elements, random rest curvature and velocity
-  ragged_c2          4096 rods, n log-uniform in [16, 1024], random materials,
-                     constant tip forces
-  contact_c4         rods lying above the plane z = 0 under gravity (contact)
-  serving_library    template rods for the serving workload, and
-  serving_requests   requests of 1-64 template rods with horizons of 1e3-1e5 steps
+  timoshenko_cantilever  1 rod x 100 elements, clamped, tip force (0, -15, 0)
+  uniform_1024x64        1024 rods x 64 elements, random rest curvature and velocity
+  ragged_4096            4096 rods, n log-uniform in [16, 1024], random materials,
+                         constant tip forces
+  rods_on_plane          100,000 rods x 128 elements lying above the plane z = 0
+                         under gravity (contact)
+  serving_library        template rods for the serving workload, and
+  serving_requests       requests of 1-64 template rods with horizons of 1e3-1e5 steps
 
 Material conventions: G = E / (1 + 0.99) (a Poisson ratio of 0.99 as stated
 for the Timoshenko configuration); damping rates for forces and torques are
@@ -70,14 +71,14 @@ def stable_dt(length, n, youngs, density, radius=None, shear=None, damping=None)
     return dt
 
 
-def timoshenko_c0(steps=50000):
+def timoshenko_cantilever(steps=50000):
     n, L, r, rho, E = 100, 3.0, 0.25, 5000.0, 1e6
     G = E / (1.0 + 0.99)
     st = straight_rod_state(n, r, L, [0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0], rho, E, G, 0.1, 0.1)
     dt = 0.01 * (L / n)
     clamps = [(0, st["Position"][:, 0].copy(), st["Director"][:, 0].copy())]
     forces = [(0, np.array([0.0, -15.0, 0.0]), 0, 0.0, 1.0)]
-    return Workload("timoshenko_c0", [n], [st], dt, steps, clamps=clamps, forces=forces,
+    return Workload("timoshenko_cantilever", [n], [st], dt, steps, clamps=clamps, forces=forces,
                     info=dict(length=L, radius=r, density=rho, youngs=E, shear=G, damping=0.1))
 
 
@@ -88,7 +89,7 @@ def _randomize(st, rng, kappa_std=0.5, v_std=1e-2):
     return st
 
 
-def uniform_c1(n_rods=1024, n=64, steps=10000, seed=1001):
+def uniform_1024x64(n_rods=1024, n=64, steps=10000, seed=1001):
     rng = np.random.default_rng(seed)
     L, r, rho, E = 3.0, 0.25, 5000.0, 1e6
     G = E / (1.0 + 0.99)
@@ -97,11 +98,11 @@ def uniform_c1(n_rods=1024, n=64, steps=10000, seed=1001):
         st = straight_rod_state(n, r, L, [0.0, 1.0 * i, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0], rho, E, G, 0.1, 0.1)
         states.append(_randomize(st, rng))
     dt = 0.01 * (L / n)
-    return Workload("uniform_c1", [n] * n_rods, states, dt, steps,
+    return Workload("uniform_1024x64", [n] * n_rods, states, dt, steps,
                     info=dict(length=L, radius=r, density=rho, youngs=E, shear=G, damping=0.1))
 
 
-def ragged_c2(n_rods=4096, steps=10000, seed=2002, n_min=16, n_max=1024):
+def ragged_4096(n_rods=4096, steps=10000, seed=2002, n_min=16, n_max=1024):
     rng = np.random.default_rng(seed)
     n_elems = np.rint(np.exp(rng.uniform(np.log(n_min), np.log(n_max), n_rods))).astype(int)
     n_elems = np.clip(n_elems, n_min, n_max)
@@ -119,7 +120,7 @@ def ragged_c2(n_rods=4096, steps=10000, seed=2002, n_min=16, n_max=1024):
         forces.append((i, F[i].copy(), 0, 0.0, 1.0))
     dts = stable_dt(L, n_elems, E, rho)
     rod_dt = stable_dt(L, n_elems, E, rho, r, E / (1.0 + 0.99), gam)
-    return Workload("ragged_c2", [int(x) for x in n_elems], states, float(dts.min()), steps, forces=forces,
+    return Workload("ragged_4096", [int(x) for x in n_elems], states, float(dts.min()), steps, forces=forces,
                     info=dict(rod_dt_min=float(dts.min()), rod_dt_max=float(dts.max()), rod_dt=rod_dt))
 
 
@@ -185,7 +186,7 @@ class UniformRods:
             block.set_field(tag, buf[:, :full.shape[1]])
 
 
-def contact_c4(n_rods=100000, n=128, steps=2000, seed=4004, k=1e3, c=10.0, mu=0.3, ct=10.0):
+def rods_on_plane(n_rods=100000, n=128, steps=2000, seed=4004, k=1e3, c=10.0, mu=0.3, ct=10.0):
     """Rods of length 1 m and radius 0.01 m (rho 1000, E 1e6, damping 0.1)
     lying along x at z = radius, 5 cm apart in y, so that they touch the plane
     z = 0 (contact surface z0 + offset with offset = radius); rest curvature
@@ -202,7 +203,7 @@ def contact_c4(n_rods=100000, n=128, steps=2000, seed=4004, k=1e3, c=10.0, mu=0.
     offsets[:, 1] = 0.05 * np.arange(n_rods)
     states = UniformRods(base, offsets, rng.normal(0.0, 0.5, (n_rods, 3, n - 1)))
     dt = stable_dt(L, n, E, rho)
-    return Workload("contact_c4", [n] * n_rods, states, float(dt), steps, gravity=(0.0, 0.0, -9.81),
+    return Workload("rods_on_plane", [n] * n_rods, states, float(dt), steps, gravity=(0.0, 0.0, -9.81),
                     contact=(k, c, mu, ct, 0.0, r), info=dict(length=L, radius=r, density=rho, youngs=E))
 
 

@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 
 from elasticapp_amd import FP32_BULK, FP64
-from elasticapp_amd.workloads import build_block, timoshenko_c0, uniform_c1
+from elasticapp_amd.workloads import build_block, timoshenko_cantilever, uniform_1024x64
 from tests.gpu_helpers import STATE, oracle_system, rel_state_error
 
 pytestmark = pytest.mark.gpu
@@ -27,7 +27,7 @@ THREADS = max(1, min(16, len(os.sched_getaffinity(0))))
 
 
 def test_timoshenko_against_oracle():
-    wl = timoshenko_c0(steps=50000)
+    wl = timoshenko_cantilever(steps=50000)
     fused = build_block(wl)
     fused.step_fused(wl.steps, FP64)
     sysm = oracle_system(wl, max_block_elems=1024)
@@ -48,7 +48,7 @@ def test_timoshenko_against_oracle():
 
 
 def test_uniform_batch_against_oracle():
-    wl = uniform_c1(steps=10000)
+    wl = uniform_1024x64(steps=10000)
     fused = build_block(wl)
     fused.step_fused(wl.steps, FP64)
     assert not fused.check_finite().any()
@@ -81,7 +81,7 @@ def test_cpp_driver_matches_oracle(tmp_path):
     """The C++ host layer (erod.hpp): the Timoshenko driver's tip after 2,000
     steps on the op-by-op and fused paths, and resumed from a checkpoint
     halfway, against the oracle on the same configuration."""
-    wl = timoshenko_c0(steps=2000)
+    wl = timoshenko_cantilever(steps=2000)
     sysm = oracle_system(wl, max_block_elems=1024)
     sysm.step(wl.dt, wl.steps)
     tip = sysm.rod_field(0, "Position")[:, -1]

@@ -14,14 +14,14 @@ import numpy as np
 import pytest
 
 from elasticapp_amd import FP32_BULK, FP64
-from elasticapp_amd.workloads import Workload, build_block, contact_c4
+from elasticapp_amd.workloads import Workload, build_block, rods_on_plane
 from tests.gpu_helpers import STATE, oracle_system, rel_state_error
 
 pytestmark = pytest.mark.gpu
 
 
 def test_contact_cut_matches_oracle():
-    wl = contact_c4(n_rods=48, n=128, steps=300)
+    wl = rods_on_plane(n_rods=48, n=128, steps=300)
     sysm = oracle_system(wl)
     sysm.step(wl.dt, wl.steps)
     a = build_block(wl)
@@ -47,7 +47,7 @@ def test_tilted_plane_matches_oracle():
     result of the level batch."""
     from scipy.spatial.transform import Rotation
     from tests.gpu_helpers import rotated
-    base = contact_c4(n_rods=48, n=128, steps=300)
+    base = rods_on_plane(n_rods=48, n=128, steps=300)
     R = Rotation.from_rotvec([0.3, -0.5, 0.2]).as_matrix()
     wl = rotated(base, R)
     sysm = oracle_system(wl)
@@ -78,22 +78,22 @@ STEPS = 200
 
 
 @pytest.fixture(scope="module")
-def c4():
-    return contact_c4(steps=STEPS)
+def plane_batch():
+    return rods_on_plane(steps=STEPS)
 
 
-def test_full_contact_batch(c4):
-    blk = build_block(c4)
+def test_full_contact_batch(plane_batch):
+    blk = build_block(plane_batch)
     blk.step_fused_graph(STEPS, 100, FP32_BULK)
     assert not blk.check_finite().any()
     x = blk.get_field("Position")
     # static deflection m g / k = (rho A L / n) g / k ~ 2.4e-5 m; allow 40x for the dynamics
-    r = c4.info["radius"]
-    m = c4.info["density"] * np.pi * r * r * c4.info["length"] / 128
+    r = plane_batch.info["radius"]
+    m = plane_batch.info["density"] * np.pi * r * r * plane_batch.info["length"] / 128
     sink = r - x[2, np.concatenate([np.arange(s, s + 129) for s in blk.starts[:2000]])].min()
     assert sink < 40 * m * 9.81 / 1e3, sink
     for rod in (0, 77777, 99999):
-        one = Workload("one", [128], [c4.states[rod]], c4.dt, STEPS, gravity=c4.gravity, contact=c4.contact)
+        one = Workload("one", [128], [plane_batch.states[rod]], plane_batch.dt, STEPS, gravity=plane_batch.gravity, contact=plane_batch.contact)
         b1 = build_block(one)
         b1.step_fused_graph(STEPS, 100, FP32_BULK)
         for t in STATE:

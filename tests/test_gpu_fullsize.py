@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from elasticapp_amd import FP32_BULK, FP64
-from elasticapp_amd.workloads import Workload, build_block, ragged_c2
+from elasticapp_amd.workloads import Workload, build_block, ragged_4096
 from tests.gpu_helpers import STATE
 
 pytestmark = pytest.mark.gpu
@@ -20,28 +20,28 @@ STEPS = 200
 
 
 @pytest.fixture(scope="module")
-def c2():
-    return ragged_c2(steps=STEPS)
+def ragged():
+    return ragged_4096(steps=STEPS)
 
 
 @pytest.fixture(scope="module")
-def fused64(c2):
-    blk = build_block(c2)
+def fused64(ragged):
+    blk = build_block(ragged)
     blk.step_fused(STEPS, FP64)
     yield blk
     blk.close()
 
 
-def test_c2_layout(c2):
-    n = np.array(c2.n_elems)
+def test_ragged_layout(ragged):
+    n = np.array(ragged.n_elems)
     assert len(n) == 4096 and n.min() >= 16 and n.max() <= 1024
     assert 0.9e6 < n.sum() < 1.1e6
 
 
-def test_c2_fused_fp64_equals_op_by_op(c2, fused64):
+def test_ragged_fused_fp64_equals_op_by_op(ragged, fused64):
     assert not fused64.check_finite().any()
-    ops = build_block(c2)
-    ops.step(c2.dt, STEPS)
+    ops = build_block(ragged)
+    ops.step(ragged.dt, STEPS)
     for t in STATE:
         a, b = fused64.get_field(t), ops.get_field(t)
         err = np.abs(a - b).max() / np.abs(b).max()
@@ -49,8 +49,8 @@ def test_c2_fused_fp64_equals_op_by_op(c2, fused64):
     ops.close()
 
 
-def test_c2_fp32_bulk_close_to_fp64(c2, fused64):
-    blk = build_block(c2)
+def test_ragged_fp32_bulk_close_to_fp64(ragged, fused64):
+    blk = build_block(ragged)
     blk.step_fused_graph(STEPS, 100, FP32_BULK)
     assert not blk.check_finite().any()
     x32, x64 = blk.get_field("Position"), fused64.get_field("Position")
@@ -59,9 +59,9 @@ def test_c2_fp32_bulk_close_to_fp64(c2, fused64):
     blk.close()
 
 
-def test_c2_rods_alone_equal_in_batch(c2, fused64):
-    for r in (0, 1234, 4095, int(np.argmax(c2.n_elems))):
-        one = Workload("one", [c2.n_elems[r]], [c2.states[r]], c2.dt, STEPS, forces=[(0,) + tuple(c2.forces[r][1:])])
+def test_ragged_rods_alone_equal_in_batch(ragged, fused64):
+    for r in (0, 1234, 4095, int(np.argmax(ragged.n_elems))):
+        one = Workload("one", [ragged.n_elems[r]], [ragged.states[r]], ragged.dt, STEPS, forces=[(0,) + tuple(ragged.forces[r][1:])])
         blk = build_block(one)
         blk.step_fused(STEPS, FP64)
         for t in STATE:
@@ -74,7 +74,7 @@ def test_fp32_bulk_error_after_ten_thousand_steps():
     10,000 steps of a 256-rod batch from the same ragged generator, each rod
     with its own stable dt, every rod's positions stay within 1e-6 of its
     length of the FP64 trajectory."""
-    wl = ragged_c2(n_rods=256, steps=10000)
+    wl = ragged_4096(n_rods=256, steps=10000)
     a, b = build_block(wl), build_block(wl)
     for blk in (a, b):
         blk.set_rod_dt(wl.info["rod_dt"])

@@ -266,9 +266,9 @@ def test_free_rods_amplify_a_last_bit_change():
     one velocity component of each rod grows to ~1e-9 of the angular
     velocities over 10,000 steps while positions stay within ~1e-14.  Bounds a GPU run whose
     transcendentals differ from the host's in the last bit can be held to."""
-    from elasticapp_amd.workloads import Workload, uniform_c1
+    from elasticapp_amd.workloads import Workload, uniform_1024x64
     from tests.gpu_helpers import oracle_system
-    wl = uniform_c1(steps=10000)
+    wl = uniform_1024x64(steps=10000)
     a = oracle_system(Workload("a", wl.n_elems[:32], wl.states[:32], wl.dt, 0), max_block_elems=1024)
     st2 = [dict(s) for s in wl.states[:32]]
     for st in st2:
@@ -293,9 +293,9 @@ def test_plane_contact_is_rotation_equivariant():
     plane, both forces; CapsulePlane.hpp:73-80, LinearSpringDashpot.cpp:49-71).
     Taking the slip in the x-y plane instead of the tilted plane moves the
     rotated rods by ~1e-4 of their length here."""
-    from elasticapp_amd.workloads import contact_c4
+    from elasticapp_amd.workloads import rods_on_plane
     from tests.gpu_helpers import oracle_system, rotated
-    wl = contact_c4(n_rods=4, n=16, steps=400)
+    wl = rods_on_plane(n_rods=4, n=16, steps=400)
     # sliding rods: the friction law acts along their slip
     for st in wl.states:
         st["Velocity"] = st["Velocity"] + np.array([[0.3], [-0.2], [0.0]])

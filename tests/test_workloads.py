@@ -4,7 +4,7 @@ seeded builders are reproducible."""
 import numpy as np
 
 import elasticapp_amd as ea
-from elasticapp_amd.workloads import UniformRods, contact_c4, ragged_c2
+from elasticapp_amd.workloads import UniformRods, rods_on_plane, ragged_4096
 
 
 class FakeBlock:
@@ -32,7 +32,7 @@ class FakeBlock:
 
 
 def test_fill_block_matches_per_rod_assembly():
-    wl = contact_c4(n_rods=7, n=12)
+    wl = rods_on_plane(n_rods=7, n=12)
     assert isinstance(wl.states, UniformRods)
     a, b = FakeBlock(wl.n_elems), FakeBlock(wl.n_elems)
     ea.assemble(a, wl.states)            # whole-field path
@@ -47,7 +47,7 @@ def test_fill_block_matches_per_rod_assembly():
 
 
 def test_contact_c4_geometry_and_dt():
-    wl = contact_c4(n_rods=5, n=128)
+    wl = rods_on_plane(n_rods=5, n=128)
     st = wl.states[4]
     assert np.allclose(st["Position"][2], 0.01) and np.allclose(st["Position"][1], 0.2)
     assert np.isclose(st["ReferenceElementLength"].sum(), 1.0)
@@ -55,13 +55,13 @@ def test_contact_c4_geometry_and_dt():
     assert np.isclose(wl.dt, 0.01 / 128 * np.sqrt(200.0) / np.sqrt(1e3))
     assert wl.contact == (1e3, 10.0, 0.3, 10.0, 0.0, 0.01) and wl.gravity == (0.0, 0.0, -9.81)
     # same seed, same rods; rods differ in their rest curvature
-    again = contact_c4(n_rods=5, n=128)
+    again = rods_on_plane(n_rods=5, n=128)
     assert np.array_equal(again.states[2]["ReferenceCurvature"], wl.states[2]["ReferenceCurvature"])
     assert not np.array_equal(wl.states[1]["ReferenceCurvature"], wl.states[2]["ReferenceCurvature"])
 
 
 def test_ragged_c2_is_seeded():
-    a, b = ragged_c2(n_rods=64), ragged_c2(n_rods=64)
+    a, b = ragged_4096(n_rods=64), ragged_4096(n_rods=64)
     assert a.n_elems == b.n_elems and a.dt == b.dt
     assert np.array_equal(a.states[10]["Velocity"], b.states[10]["Velocity"])
     assert min(a.n_elems) >= 16 and max(a.n_elems) <= 1024
