@@ -7,6 +7,7 @@
 // 291-321) plus hook descriptors; steps can be spread over a std::thread pool,
 // one task per block as tbb::parallel_for_each does (elastica/Parallel/for_each.hpp:90-99).
 #include <atomic>
+#include <cfenv>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -18,6 +19,16 @@ using T = double;
 using Sys = oracle::System<T>;
 
 namespace {
+  // The reference's test runner enables floating-point exceptions, so an
+  // invalid operation, a division by zero or an overflow on the path aborts
+  // its tests (tests/Unit/RunTests.cpp).  Here the calls that compute on the
+  // calling thread record the flags they raised instead; the tests read them.
+  int g_fp_flags = 0;
+  struct FpWatch {
+    FpWatch() { std::feclearexcept(FE_ALL_EXCEPT); }
+    ~FpWatch() { g_fp_flags |= std::fetestexcept(FE_INVALID | FE_DIVBYZERO | FE_OVERFLOW); }
+  };
+
   struct Pool {
     // fan out one task per block, join before returning (the TBB barrier)
     template <typename F>
@@ -126,17 +137,31 @@ int oracle_initialize_straight_rod(void* h, uint64_t rod, const double* params16
 
 // Single component calls on every block (the FullRodTest InitializeStraightRod path).
 int oracle_compute_internal_forces(void* h) {
+  FpWatch w;
   for (auto& b : static_cast<Sys*>(h)->blocks) oracle::compute_internal_forces(b);
   return 0;
 }
 int oracle_compute_internal_torques(void* h) {
+  FpWatch w;
   for (auto& b : static_cast<Sys*>(h)->blocks) oracle::compute_internal_torques(b);
   return 0;
 }
 int oracle_update_acceleration(void* h) {
+  FpWatch w;
   for (auto& b : static_cast<Sys*>(h)->blocks) oracle::update_acceleration(b);
   return 0;
 }
+
+// the floating-point exception flags (FE_INVALID | FE_DIVBYZERO | FE_OVERFLOW)
+// raised by the watched calls since the last read; steps on one thread only
+int oracle_fp_flags(int clear) {
+  const int f = g_fp_flags;
+  if (clear) g_fp_flags = 0;
+  return f;
+}
+int oracle_fe_invalid(void) { return FE_INVALID; }
+int oracle_fe_divbyzero(void) { return FE_DIVBYZERO; }
+int oracle_fe_overflow(void) { return FE_OVERFLOW; }
 int oracle_kinematic_update(void* h, double dt) {
   for (auto& b : static_cast<Sys*>(h)->blocks) oracle::kinematic_update(b, dt);
   return 0;
@@ -207,6 +232,7 @@ void oracle_set_time(void* h, double t) { static_cast<Sys*>(h)->time = t; }
 int oracle_step(void* h, double dt, uint64_t n, int nthreads) {
   auto* s = static_cast<Sys*>(h);
   auto fe = [&](auto&& fn) { Pool::for_each(s->blocks, nthreads, fn); };
+  FpWatch w;
   for (uint64_t i = 0; i < n; ++i) oracle::step(*s, dt, fe);
   return 0;
 }

@@ -62,6 +62,9 @@ def _lib(path=None):
         L.oracle_set_gravity.argtypes = [vp, dp]
         L.oracle_set_plane_contact.argtypes = [vp] + [ctypes.c_double] * 6
         L.oracle_set_plane_normal.argtypes = [vp, dp]
+        for name in ("oracle_fp_flags", "oracle_fe_invalid", "oracle_fe_divbyzero", "oracle_fe_overflow"):
+            getattr(L, name).restype = ctypes.c_int
+        L.oracle_fp_flags.argtypes = [ctypes.c_int]
         L.oracle_ramp_value.restype = ctypes.c_double
         L.oracle_ramp_value.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.oracle_time.restype = ctypes.c_double
@@ -236,3 +239,14 @@ def so3_add_assign(Q, u):
 def ramp_value(kind, t, start, interval):
     """The ramp factor of an end force at time t (0 none, 1 linear, 2 sin)."""
     return _lib().oracle_ramp_value(int(kind), float(t), float(start), float(interval))
+
+
+def fp_exceptions(clear=True):
+    """Floating-point exceptions (invalid, divide-by-zero, overflow) raised by
+    the oracle's steps and force evaluations on the calling thread since the
+    last read, by name."""
+    L = _lib()
+    f = L.oracle_fp_flags(1 if clear else 0)
+    names = (("invalid", L.oracle_fe_invalid()), ("divide-by-zero", L.oracle_fe_divbyzero()),
+             ("overflow", L.oracle_fe_overflow()))
+    return [n for n, bit in names if f & bit]

@@ -56,9 +56,13 @@ def _step_cases():
 
 @pytest.mark.parametrize("case,section", _step_cases())
 def test_golden_steps(case, section):
+    """Every variable the reference compares, at its tolerance; and, as its
+    test runner enables floating-point exceptions, none raised."""
     sysm = _build(case)
+    pyoracle.fp_exceptions()
     for _ in range(section["steps"]):
         sysm.step(case["dt"], 1)
+    assert pyoracle.fp_exceptions() == []
     bad = _compare(sysm, section["expected"], section["epsilon"])
     assert not bad, "\n".join(bad)
 
@@ -70,9 +74,11 @@ def test_golden_single_evaluation():
     sysm = _build(case)
     sec = case["sections"][0]
     assert sec["calls"] == ["compute_internal_forces", "compute_internal_torques", "update_acceleration"]
+    pyoracle.fp_exceptions()
     sysm.compute_internal_forces()
     sysm.compute_internal_torques()
     sysm.update_acceleration()
+    assert pyoracle.fp_exceptions() == []
     bad = _compare(sysm, sec["expected"], sec["epsilon"])
     assert not bad, "\n".join(bad)
 

@@ -350,3 +350,33 @@ def test_sin_ramp_is_the_smooth_step():
         got = pyoracle.ramp_value(2, s + tau * T, s, T)
         want = quad(lambda u: 1.0 - np.cos(2.0 * np.pi * u), 0.0, tau, epsabs=1e-15)[0]
         assert abs(got - want) <= 1e-14, tau
+
+
+# --- floating-point exceptions (the reference's test runner traps them) ---------------
+
+def test_no_floating_point_exceptions_on_the_batches():
+    """No invalid operation, division by zero or overflow in steps of a ragged
+    batch (2- to 300-element rods, hooks) and of the contact batch, as the
+    reference's tests run with floating-point exceptions enabled."""
+    from elasticapp_amd.workloads import rods_on_plane
+    from tests.gpu_helpers import oracle_system, random_workload
+    pyoracle.fp_exceptions()
+    wl = random_workload(40, 2, 300, seed=7, steps=200)
+    s = oracle_system(wl)
+    s.step(wl.dt, wl.steps)
+    assert pyoracle.fp_exceptions() == []
+    wl = rods_on_plane(n_rods=8, n=32, steps=300)
+    s = oracle_system(wl)
+    s.step(wl.dt, wl.steps)
+    assert pyoracle.fp_exceptions() == []
+
+
+def test_a_zero_rest_length_is_reported():
+    """The watch sees what the reference would trap: a zero reference element
+    length divides by zero in the dilatation e = l / l0."""
+    s = pyoracle.System([6])
+    s.init_straight_rod(0, 0.01, 1.0, [0, 0, 0], [1, 0, 0], [0, 1, 0], 1e3, 1e6, 4e5, 0.1, 0.1)
+    s.rod_field(0, "ReferenceElementLength")[0, 2] = 0.0
+    pyoracle.fp_exceptions()
+    s.step(1e-5, 1)
+    assert "divide-by-zero" in pyoracle.fp_exceptions()
