@@ -399,3 +399,12 @@ def test_device_ramp_matches_the_reference_known_answers(ramp):
         want = pyoracle.ramp_value(ramp, t, s, T)
         assert np.abs(got - want * np.array([1.0, -2.0, 0.5])).max() <= 2e-16 * 2.0, (t, got, want)
     blk.close()
+
+
+def test_batch_length_pins_two_ghost_elements_per_gap():
+    """Test_CosseratRod.py:13-22 on the device batch: 4 * 10 + 3 * 2 = 46
+    element slots, rods starting 12 slots apart."""
+    blk = DeviceBlock([10] * 4)
+    assert blk.element_slots == 46 and blk.starts == [0, 12, 24, 36]
+    assert blk.get_field("Position").shape == (3, 47) and blk.get_field("ReferenceCurvature").shape == (3, 45)
+    blk.close()

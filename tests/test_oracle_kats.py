@@ -380,3 +380,14 @@ def test_a_zero_rest_length_is_reported():
     pyoracle.fp_exceptions()
     s.step(1e-5, 1)
     assert "divide-by-zero" in pyoracle.fp_exceptions()
+
+
+def test_block_length_pins_two_ghost_elements_per_gap():
+    """Test_CosseratRod.py:13-22: four 10-element rods in one block take
+    4 * 10 + 3 * 2 = 46 element slots, each rod starting 12 slots after the
+    previous one (s_{i+1} = s_i + n_i + 2)."""
+    s = pyoracle.System([10] * 4, max_block_elems=1024)
+    assert s.n_blocks == 1
+    assert s.block_field(0, "ElementLength").shape == (1, 46)
+    assert s.block_field(0, "Position").shape == (3, 47)
+    assert [s.location(r) for r in range(4)] == [(0, 0), (0, 12), (0, 24), (0, 36)]
