@@ -15,7 +15,10 @@ rod state on chip for all steps of a launch, its HBM traffic is far below the
 algorithmic bytes and the fraction can exceed 1; `roofline_streaming` prices
 the same kernel at one step per launch (state through HBM every step), and
 `traffic` holds the HBM bytes per launch measured with rocprofv3 counters
-(committed under profiles/, matched on the tile shape).
+(committed under profiles/, matched on the tile shape).  `roofline_valu`
+prices the same launch by what bounds it with the state on chip: its VALU
+instructions (the counter summary's instructions per wave and step) against
+one wave64 VALU instruction per CU per cycle.
 """
 import argparse
 import json
