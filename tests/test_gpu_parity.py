@@ -276,6 +276,30 @@ def test_fused_graph_replay_equals_direct():
             assert np.array_equal(a.get_rod(t, r), b.get_rod(t, r)), (r, t)
 
 
+@pytest.mark.parametrize("precision", [FP32_BULK, FP64])
+def test_graph_replays_with_eager_work_between(precision):
+    """One captured chunk replayed four times, each replay followed by eager
+    work on the device (another batch's fused steps, the finiteness check with
+    its clear, downloads): every replay equals the eager path advanced by the
+    same steps."""
+    wl = random_workload(20, 2, 120, seed=19, steps=0)
+    a, b = device_block(wl), device_block(wl)
+    for blk in (a, b):
+        blk.set_fused_config(0, 0, 20)
+    other = device_block(random_workload(5, 4, 30, seed=20, steps=0))
+    for rep in range(4):
+        a.step_fused(20, precision)
+        b.step_fused_graph(20, 20, precision)
+        other.step_fused(7, FP64)
+        assert not b.check_finite().any()
+        for r in range(len(wl.n_elems)):
+            for t in STATE:
+                assert np.array_equal(a.get_rod(t, r), b.get_rod(t, r)), (rep, r, t)
+        assert np.array_equal(a.rod_time(), b.rod_time()), rep
+    for blk in (a, b, other):
+        blk.close()
+
+
 def test_rod_alone_equals_rod_in_batch():
     wl = random_workload(12, 2, 90, seed=17, steps=40)
     full = device_block(wl)
