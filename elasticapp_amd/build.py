@@ -9,6 +9,7 @@ FP32-bulk kernel's only fused multiply-adds are the explicit ones in its
 source, so its results do not depend on the tile shape.  Packed FP32
 instructions are disabled for all device code.
 """
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -44,14 +45,20 @@ def hipcc():
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     cc = hipcc()
-    objs = []
+    objs, cmds = [], []
     for src, extra in SOURCES.items():
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
-        cmd = [cc] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append([cc] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    # the translation units compile independently: one hipcc per host core, at most 8
+    workers = max(1, min(8, len(os.sched_getaffinity(0)), len(cmds)))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as ex:
+        def run(cmd):
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            subprocess.run(cmd, check=True)
+        for f in [ex.submit(run, c) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     cmd = [cc, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
     if verbose:
