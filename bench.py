@@ -55,15 +55,16 @@ def make_block(wl):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_fp32_ragged4096_pmc.md")
 
 
-def measured_traffic(lanes, epl, spl):
+def measured_traffic(lanes, epl, spl, tag="traffic"):
     """HBM bytes per launch of the FP32-bulk kernel at this tile shape, from
     the rocprofv3 FETCH_SIZE / WRITE_SIZE summary (lines of the form
-    `traffic lanes=L epl=P spl=S bytes_per_launch=B`), or None."""
+    `traffic lanes=L epl=P spl=S bytes_per_launch=B` for the ragged batch,
+    `traffic_contact ...` for the rods on a plane), or None."""
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
                 w = line.strip().strip("`").split()
-                if len(w) == 5 and w[0] == "traffic":
+                if len(w) == 5 and w[0] == tag:
                     kv = dict(x.split("=", 1) for x in w[1:])
                     if (int(kv["lanes"]), int(kv["epl"]), int(kv["spl"])) == (lanes, epl, spl):
                         return float(kv["bytes_per_launch"])
@@ -201,7 +202,8 @@ def contact_leg(args, torch):
             "steps_per_launch": spl, "min_z": zmin,
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "kernel_ms_per_launch": kms, "launches": launches,
-                         "algorithmic_bytes_per_launch": bytes_launch},
+                         "algorithmic_bytes_per_launch": bytes_launch,
+                         "traffic": measured_traffic(lanes, epl, spl, tag="traffic_contact")},
             "fp64_plain": {"value": vp, "steps": args.contact_plain_steps}, "ratio_fp32bulk_vs_plain_fp64": v / vp}
 
 
