@@ -53,15 +53,16 @@ def make_block(wl):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_fp32_ragged4096_pmc.md")
+CONTACT_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_fp32_rods_on_plane_pmc.md")
 
 
-def measured_traffic(lanes, epl, spl, tag="traffic"):
+def measured_traffic(lanes, epl, spl, tag="traffic", path=TRAFFIC_FILE):
     """HBM bytes per launch of the FP32-bulk kernel at this tile shape, from
     the rocprofv3 FETCH_SIZE / WRITE_SIZE summary (lines of the form
     `traffic lanes=L epl=P spl=S bytes_per_launch=B` for the ragged batch,
     `traffic_contact ...` for the rods on a plane), or None."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             for line in f:
                 w = line.strip().strip("`").split()
                 if len(w) == 5 and w[0] == tag:
@@ -203,7 +204,8 @@ def contact_leg(args, torch):
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "kernel_ms_per_launch": kms, "launches": launches,
                          "algorithmic_bytes_per_launch": bytes_launch,
-                         "traffic": measured_traffic(lanes, epl, spl, tag="traffic_contact")},
+                         "traffic": measured_traffic(lanes, epl, spl, tag="traffic_contact",
+                                                     path=CONTACT_TRAFFIC_FILE)},
             "fp64_plain": {"value": vp, "steps": args.contact_plain_steps}, "ratio_fp32bulk_vs_plain_fp64": v / vp}
 
 

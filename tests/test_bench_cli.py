@@ -32,7 +32,7 @@ def test_committed_counter_summary_feeds_the_rooflines():
     # per element plus FP32 constants, for the 994,351 elements of the batch
     assert 0.3e9 < t100 < 0.5e9 and 0.3e9 < t1 < 0.5e9
     # the contact batch: 12.8 M elements, one pass over state and constants per launch
-    tc = bench.measured_traffic(512, 2, 100, tag="traffic_contact")
+    tc = bench.measured_traffic(512, 2, 100, tag="traffic_contact", path=bench.CONTACT_TRAFFIC_FILE)
     assert tc is not None and 4e9 < tc < 6e9
     waves, ipws = bench.measured_valu(1024, 2, 100)
     assert waves % 8 == 0 and 500 < ipws < 1200
