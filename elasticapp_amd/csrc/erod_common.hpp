@@ -87,7 +87,6 @@ namespace erod {
     int elements_per_thread = 0;  // requested elements per thread (0 = any)
     int steps_per_launch = 0;     // requested (0 = 100)
     int shape = -1;               // chosen index into fused::kShapes
-    int chosen_lanes = 0;
     uint64_t n_tiles = 0;
     uint32_t* d_tile_rods = nullptr;  // rod ids in tile order
     uint32_t* d_tile_lane0 = nullptr; // first lane of each rod within its tile
@@ -141,7 +140,6 @@ struct erod_batch {
   uint64_t graph_chunk = 0;
   // timing of the dominant kernel
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double last_kernel_ms = 0.0;
   uint64_t last_launches = 0;
   // scratch
   int* d_flags = nullptr;

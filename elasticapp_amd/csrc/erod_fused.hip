@@ -234,7 +234,6 @@ namespace erod {
     EROD_HIP(hipStreamSynchronize(b->stream));
     b->plan.n_tiles = tiles.size();
     b->plan.shape = si;
-    b->plan.chosen_lanes = static_cast<int>(cap);
     b->plan.valid = true;
     b->graph_precision = -1;
     return EROD_OK;
