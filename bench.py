@@ -416,8 +416,8 @@ def main():
         blk32.set_fused_config(0, 0, args.chunk)
         # one trial launch of `chunk` steps per tile shape
         tune_ms = blk32.autotune(FP32_BULK, args.chunk)
-        log("autotuned shape (lanes, elements/thread, steps/launch) = %s, %.4f ms/step"
-            % (blk32.fused_config(), tune_ms))
+        log("autotuned shape (lanes, elements/thread, steps/launch) = %s, second group %s, %.4f ms/step"
+            % (blk32.fused_config(), blk32.fused_split(), tune_ms))
 
     def sync32():
         blk32.synchronize()
@@ -453,7 +453,9 @@ def main():
     sync64()
     el64 = timed(lambda: blk64.step_fused_graph(args.steps, args.chunk, FP64), sync64)
     v64 = elems * args.steps / el64
-    log("fp64 fused: %.3f s -> %.3e element-steps/s" % (el64, v64))
+    plan64 = {"tile": list(blk64.fused_config()), "second_group": list(blk64.fused_split())}
+    log("fp64 fused: %.3f s -> %.3e element-steps/s (tiles %s, second group %s)"
+        % (el64, v64, plan64["tile"], plan64["second_group"]))
     # error of the FP32-bulk positions against FP64 after the same steps:
     # per rod max |x32 - x64| over its nodes, relative to the rod length
     x32 = blk32.get_field("Position")
@@ -547,7 +549,9 @@ def main():
         "rods_on_plane": contact,
         "serving": serving,
         "cpu_baseline": cpu,
-        "fp64_fused": {"value": v64, "ms_per_step": el64 * 1e3 / args.steps},
+        "fp64_fused": {"value": v64, "ms_per_step": el64 * 1e3 / args.steps,
+                       "plan": {"tile (lanes, elements/thread, steps/launch)": plan64["tile"],
+                                "second group (lanes, elements/thread, tiles)": plan64["second_group"]}},
         "fp64_plain": {"value": vp, "steps": args.plain_steps},
         "ratio_fp32bulk_vs_plain_fp64": v32 / vp,
         "fp32_bulk_position_err_vs_fp64": {"value": pos_err, "metric": "max over rods of max |x32 - x64| / rod length",

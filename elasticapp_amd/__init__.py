@@ -92,6 +92,8 @@ def load_library():
         "erod_set_rod_time": (i32, [vp, P(dbl)]),
         "erod_set_fused_config": (i32, [vp, i32, i32, i32]),
         "erod_get_fused_config": (i32, [vp, P(i32), P(i32), P(i32)]),
+        "erod_set_fused_split": (i32, [vp, i32, i32]),
+        "erod_get_fused_split": (i32, [vp, P(i32), P(i32), P(u64)]),
         "erod_autotune_fused": (i32, [vp, i32, i32, P(dbl)]),
         "erod_step_fused": (i32, [vp, i32, u64]),
         "erod_step_fused_graph": (i32, [vp, i32, u64, u64]),
@@ -301,6 +303,17 @@ class DeviceBlock:
     def set_fused_config(self, lanes_per_tile=0, elements_per_thread=0, steps_per_launch=0):
         _check(_LIB.erod_set_fused_config(self._h, int(lanes_per_tile), int(elements_per_thread),
                                           int(steps_per_launch)))
+
+    def set_fused_split(self, lanes_per_tile=0, elements_per_thread=0):
+        """Rods of at most `lanes_per_tile` elements that find no room beside
+        longer rods run in a second group of tiles of that shape (0 = off)."""
+        _check(_LIB.erod_set_fused_split(self._h, int(lanes_per_tile), int(elements_per_thread)))
+
+    def fused_split(self):
+        """(lanes, elements per thread, tiles) of the current plan's second group."""
+        a, b, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_uint64()
+        _check(_LIB.erod_get_fused_split(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(t)))
+        return a.value, b.value, t.value
 
     def fused_config(self):
         """(lanes_per_tile, elements_per_thread, steps_per_launch) of the current plan."""

@@ -239,6 +239,9 @@ int erod_batch_destroy(erod_batch* b) {
   hipFree(b->plan.d_tiles);
   hipFree(b->plan.d_tile_rods);
   hipFree(b->plan.d_tile_lane0);
+  hipFree(b->plan.d_tiles2);
+  hipFree(b->plan.d_tile_rods2);
+  hipFree(b->plan.d_tile_lane02);
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);
   if (b->stream) hipStreamDestroy(b->stream);
@@ -631,6 +634,27 @@ int erod_set_fused_config(erod_batch* b, int lanes_per_tile, int elements_per_th
   b->plan.steps_per_launch = steps_per_launch;
   b->plan.valid = false;
   b->graph_precision = -1;
+  return EROD_OK;
+}
+
+int erod_set_fused_split(erod_batch* b, int lanes_per_tile, int elements_per_thread) {
+  if (check_batch(b) || lanes_per_tile < 0 || elements_per_thread < 0) {
+    set_error("erod_set_fused_split: negative argument");
+    return EROD_ERR_INVALID;
+  }
+  b->plan.split_lanes = lanes_per_tile;
+  b->plan.split_epl = elements_per_thread;
+  b->plan.valid = false;
+  b->graph_precision = -1;
+  return EROD_OK;
+}
+
+int erod_get_fused_split(const erod_batch* b, int* lanes, int* elements_per_thread, uint64_t* tiles) {
+  if (check_batch(b)) return EROD_ERR_INVALID;
+  const int si = b->plan.valid ? b->plan.shape2 : -1;
+  if (lanes) *lanes = si >= 0 ? fused_shape_lanes(si) : 0;
+  if (elements_per_thread) *elements_per_thread = si >= 0 ? fused_shape_epl(si) : 0;
+  if (tiles) *tiles = si >= 0 ? b->plan.n_tiles2 : 0;
   return EROD_OK;
 }
 

@@ -92,6 +92,15 @@ namespace erod {
     uint32_t* d_tile_lane0 = nullptr; // first lane of each rod within its tile
     Tile* d_tiles = nullptr;
     uint64_t rod_capacity = 0;        // entries allocated in each of the three arrays
+    // optional second group: rods of at most split_lanes elements that find no
+    // room beside longer rods run in tiles of that shape, in a second launch
+    int split_lanes = 0, split_epl = 0;  // requested (0 = no second group)
+    int shape2 = -1;                     // its index into fused::kShapes
+    uint64_t n_tiles2 = 0;
+    uint32_t* d_tile_rods2 = nullptr;
+    uint32_t* d_tile_lane02 = nullptr;
+    Tile* d_tiles2 = nullptr;
+    uint64_t rod_capacity2 = 0;
     bool valid = false;
   };
 

@@ -117,6 +117,13 @@ namespace erod {
     return -1;
   }
 
+  // One group of tiles on the host: the tiles, their rods in order and each
+  // rod's first lane; rods sent on to a second group.
+  struct Packed {
+    std::vector<Tile> tiles;
+    std::vector<uint32_t> trods, tlane0, overflow;
+  };
+
   // Best-fit-decreasing bin packing of the rods into tiles of P * NT elements:
   // each rod, longest first, goes to the tile with the least free room that
   // holds it (ties: the tile that reached that room first).  A workgroup costs
@@ -126,6 +133,112 @@ namespace erod {
   // Rod lengths and rooms are at most P * NT, so the sort is a counting sort
   // and the open tiles sit in one FIFO list per room, found through a bitset:
   // a serving pool replans its ~30,000 slots in about 0.5 ms of host time.
+  // Here `order` holds the rods, longest first, and cap is P * NT.  With
+  // spill_cap > 0, a rod of at most spill_cap elements that finds no room in an
+  // open tile goes to `overflow` instead of opening one.
+  static void pack(const erod_batch* b, uint64_t cap, const std::vector<uint32_t>& order, uint64_t spill_cap,
+                   Packed& out) {
+    // open tiles by free room: FIFO lists (head, tail, next) and a bitset of non-empty rooms
+    std::vector<int32_t> head(cap + 1, -1), tail(cap + 1, -1), next;
+    std::vector<uint64_t> nonempty(cap / 64 + 1, 0);
+    auto open = [&](uint64_t room, uint32_t t) {
+      next[t] = -1;
+      if (tail[room] < 0) head[room] = int32_t(t);
+      else next[tail[room]] = int32_t(t);
+      tail[room] = int32_t(t);
+      nonempty[room >> 6] |= uint64_t(1) << (room & 63);
+    };
+    std::vector<uint32_t> placed, tile_of, tile_rods_count;
+    for (const uint32_t r : order) {
+      const uint64_t n = b->n_active[r];
+      int64_t room = -1;  // the smallest room >= n that has a tile
+      for (uint64_t w = n >> 6; w < nonempty.size() && room < 0; ++w) {
+        uint64_t m = nonempty[w];
+        if (w == (n >> 6)) m &= ~uint64_t(0) << (n & 63);
+        if (m) room = int64_t(w * 64 + uint64_t(__builtin_ctzll(m)));
+      }
+      uint32_t t;
+      if (room < 0) {
+        if (n <= spill_cap) {
+          out.overflow.push_back(r);
+          continue;
+        }
+        t = static_cast<uint32_t>(tile_rods_count.size());
+        tile_rods_count.push_back(0);
+        next.push_back(-1);
+        if (cap > n) open(cap - n, t);
+      } else {
+        t = static_cast<uint32_t>(head[room]);
+        head[room] = next[t];
+        if (head[room] < 0) {
+          tail[room] = -1;
+          nonempty[room >> 6] &= ~(uint64_t(1) << (room & 63));
+        }
+        if (const uint64_t left = uint64_t(room) - n) open(left, t);
+      }
+      placed.push_back(r);
+      tile_of.push_back(t);
+      ++tile_rods_count[t];
+    }
+    // tiles in creation order, each with its rods in the order they were placed
+    const size_t n_tiles = tile_rods_count.size(), n_placed = placed.size();
+    out.tiles.assign(n_tiles, Tile{});
+    out.trods.assign(n_placed, 0);
+    out.tlane0.assign(n_placed, 0);
+    std::vector<uint32_t> fill(n_tiles), lane(n_tiles, 0);
+    for (uint32_t t = 0, first = 0; t < n_tiles; ++t) {
+      out.tiles[t].first = first;
+      out.tiles[t].count = tile_rods_count[t];
+      out.tiles[t].pad = 0;
+      fill[t] = first;
+      first += tile_rods_count[t];
+    }
+    for (size_t i = 0; i < n_placed; ++i) {
+      const uint32_t t = tile_of[i], k = fill[t]++;
+      out.trods[k] = placed[i];
+      out.tlane0[k] = lane[t];
+      lane[t] += static_cast<uint32_t>(b->n_active[placed[i]]);
+    }
+    for (size_t t = 0; t < n_tiles; ++t) out.tiles[t].lanes = lane[t];
+  }
+
+  // The device copy of one group.  The arrays grow to the largest plan seen
+  // (a tile or a rod entry per rod at most) and are reused when the rod set
+  // changes.
+  static int upload_group(erod_batch* b, const Packed& g, Tile*& d_tiles, uint32_t*& d_rods, uint32_t*& d_lane0,
+                          uint64_t& capacity) {
+    if (capacity < b->n_rods) {
+      EROD_HIP(hipStreamSynchronize(b->stream));
+      if (d_tiles) EROD_HIP(hipFree(d_tiles));
+      if (d_rods) EROD_HIP(hipFree(d_rods));
+      if (d_lane0) EROD_HIP(hipFree(d_lane0));
+      d_tiles = nullptr;
+      d_rods = d_lane0 = nullptr;
+      capacity = 0;
+      EROD_HIP(hipMalloc(&d_tiles, b->n_rods * sizeof(Tile)));
+      EROD_HIP(hipMalloc(&d_rods, b->n_rods * sizeof(uint32_t)));
+      EROD_HIP(hipMalloc(&d_lane0, b->n_rods * sizeof(uint32_t)));
+      capacity = b->n_rods;
+    }
+    if (!g.tiles.empty()) {
+      EROD_HIP(hipMemcpyAsync(d_tiles, g.tiles.data(), g.tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, b->stream));
+      EROD_HIP(hipMemcpyAsync(d_rods, g.trods.data(), g.trods.size() * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
+      EROD_HIP(hipMemcpyAsync(d_lane0, g.tlane0.data(), g.tlane0.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              b->stream));
+    }
+    return EROD_OK;
+  }
+
+  // the shape a requested second group uses, or -1 (none requested)
+  static int split_shape(const erod_batch* b) {
+    if (b->plan.split_lanes <= 0) return -1;
+    for (int si : {0, 1, 2, 4, 5, 3, 7, 6})  // the default order of preference
+      if (fused::kShapes[si].P * fused::kShapes[si].NT == b->plan.split_lanes &&
+          (b->plan.split_epl <= 0 || fused::kShapes[si].P == b->plan.split_epl))
+        return si;
+    return -2;
+  }
+
   static int plan_tiles(erod_batch* b, int si) {
     const uint64_t cap = uint64_t(fused::kShapes[si].P) * fused::kShapes[si].NT;
     // planned rods, longest first, stable
@@ -147,94 +260,45 @@ namespace erod {
     std::vector<uint32_t> order(n_planned);
     for (uint32_t i = 0; i < b->n_rods; ++i)
       if (planned(b, i)) order[next_pos[b->n_active[i]]++] = i;
-    // open tiles by free room: FIFO lists (head, tail, next) and a bitset of non-empty rooms
-    std::vector<int32_t> head(cap + 1, -1), tail(cap + 1, -1), next;
-    std::vector<uint64_t> nonempty(cap / 64 + 1, 0);
-    auto open = [&](uint64_t room, uint32_t t) {
-      next[t] = -1;
-      if (tail[room] < 0) head[room] = int32_t(t);
-      else next[tail[room]] = int32_t(t);
-      tail[room] = int32_t(t);
-      nonempty[room >> 6] |= uint64_t(1) << (room & 63);
-    };
-    std::vector<uint32_t> tile_of(n_planned), tile_rods_count;
-    for (uint32_t i = 0; i < n_planned; ++i) {
-      const uint64_t n = b->n_active[order[i]];
-      int64_t room = -1;  // the smallest room >= n that has a tile
-      for (uint64_t w = n >> 6; w < nonempty.size() && room < 0; ++w) {
-        uint64_t m = nonempty[w];
-        if (w == (n >> 6)) m &= ~uint64_t(0) << (n & 63);
-        if (m) room = int64_t(w * 64 + uint64_t(__builtin_ctzll(m)));
-      }
-      uint32_t t;
-      if (room < 0) {
-        t = static_cast<uint32_t>(tile_rods_count.size());
-        tile_rods_count.push_back(0);
-        next.push_back(-1);
-        if (cap > n) open(cap - n, t);
-      } else {
-        t = static_cast<uint32_t>(head[room]);
-        head[room] = next[t];
-        if (head[room] < 0) {
-          tail[room] = -1;
-          nonempty[room >> 6] &= ~(uint64_t(1) << (room & 63));
+    const int s2 = split_shape(b);
+    if (s2 == -2) {
+      set_error("plan_tiles: no tile shape of the requested split lanes and elements per thread");
+      return EROD_ERR_INVALID;
+    }
+    const uint64_t cap2 = s2 >= 0 ? uint64_t(fused::kShapes[s2].P) * fused::kShapes[s2].NT : 0;
+    Packed g1, g2;
+    pack(b, cap, order, cap2, g1);
+    if (!g1.overflow.empty()) {
+      // the second group holds rods of at most cap2 elements, and of at least P2
+      for (const uint32_t r : g1.overflow)
+        if (b->n_active[r] < uint64_t(fused::kShapes[s2].P)) {
+          set_error("plan_tiles: a rod has fewer elements than the split shape's elements per thread");
+          return EROD_ERR_INVALID;
         }
-        if (const uint64_t left = uint64_t(room) - n) open(left, t);
-      }
-      tile_of[i] = t;
-      ++tile_rods_count[t];
+      pack(b, cap2, g1.overflow, 0, g2);
     }
-    // tiles in creation order, each with its rods in the order they were placed
-    const size_t n_tiles = tile_rods_count.size();
-    std::vector<Tile> tiles(n_tiles);
-    std::vector<uint32_t> trods(n_planned), tlane0(n_planned), fill(n_tiles), lane(n_tiles, 0);
-    for (uint32_t t = 0, first = 0; t < n_tiles; ++t) {
-      tiles[t].first = first;
-      tiles[t].count = tile_rods_count[t];
-      tiles[t].pad = 0;
-      fill[t] = first;
-      first += tile_rods_count[t];
-    }
-    for (uint32_t i = 0; i < n_planned; ++i) {
-      const uint32_t t = tile_of[i], k = fill[t]++;
-      trods[k] = order[i];
-      tlane0[k] = lane[t];
-      lane[t] += static_cast<uint32_t>(b->n_active[order[i]]);
-    }
-    for (size_t t = 0; t < n_tiles; ++t) tiles[t].lanes = lane[t];
     // the kernel stages a tile's rod table in NT entries of LDS (rods have at
     // least two elements, and at least P: a feasible shape never exceeds it)
-    for (size_t t = 0; t < n_tiles; ++t)
-      if (tiles[t].count > uint32_t(fused::kShapes[si].NT)) {
+    for (const Tile& t : g1.tiles)
+      if (t.count > uint32_t(fused::kShapes[si].NT)) {
         set_error("plan_tiles: more rods in a tile than threads");
         return EROD_ERR_INVALID;
       }
-    // the plan arrays grow to the largest plan seen (a tile or a rod entry per
-    // rod at most) and are reused when the rod set changes
-    if (b->plan.rod_capacity < b->n_rods) {
-      EROD_HIP(hipStreamSynchronize(b->stream));
-      if (b->plan.d_tiles) EROD_HIP(hipFree(b->plan.d_tiles));
-      if (b->plan.d_tile_rods) EROD_HIP(hipFree(b->plan.d_tile_rods));
-      if (b->plan.d_tile_lane0) EROD_HIP(hipFree(b->plan.d_tile_lane0));
-      b->plan.d_tiles = nullptr;
-      b->plan.d_tile_rods = b->plan.d_tile_lane0 = nullptr;
-      b->plan.rod_capacity = 0;
-      EROD_HIP(hipMalloc(&b->plan.d_tiles, b->n_rods * sizeof(Tile)));
-      EROD_HIP(hipMalloc(&b->plan.d_tile_rods, b->n_rods * sizeof(uint32_t)));
-      EROD_HIP(hipMalloc(&b->plan.d_tile_lane0, b->n_rods * sizeof(uint32_t)));
-      b->plan.rod_capacity = b->n_rods;
-    }
-    if (!tiles.empty()) {
-      EROD_HIP(hipMemcpyAsync(b->plan.d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, b->stream));
-      EROD_HIP(hipMemcpyAsync(b->plan.d_tile_rods, trods.data(), trods.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
-                              b->stream));
-      EROD_HIP(hipMemcpyAsync(b->plan.d_tile_lane0, tlane0.data(), tlane0.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
-                              b->stream));
-    }
+    for (const Tile& t : g2.tiles)
+      if (t.count > uint32_t(fused::kShapes[s2].NT)) {
+        set_error("plan_tiles: more rods in a tile than threads");
+        return EROD_ERR_INVALID;
+      }
+    FusedPlan& pl = b->plan;
+    if (int rc = upload_group(b, g1, pl.d_tiles, pl.d_tile_rods, pl.d_tile_lane0, pl.rod_capacity)) return rc;
+    if (!g2.tiles.empty())
+      if (int rc = upload_group(b, g2, pl.d_tiles2, pl.d_tile_rods2, pl.d_tile_lane02, pl.rod_capacity2)) return rc;
     EROD_HIP(hipStreamSynchronize(b->stream));
-    b->plan.n_tiles = tiles.size();
-    b->plan.shape = si;
-    b->plan.valid = true;
+    pl.n_tiles = g1.tiles.size();
+    pl.shape = si;
+    pl.n_tiles2 = g2.tiles.size();
+    pl.shape2 = g2.tiles.empty() ? -1 : s2;
+    pl.valid = true;
     b->graph_precision = -1;
     return EROD_OK;
   }
@@ -310,6 +374,8 @@ namespace erod {
   // Measured choice of the tile shape: every feasible shape advances the
   // batch `trial_steps` steps from the same snapshot of x, v, Q, w and the rod
   // clocks (restored afterwards), timed with HIP events; the fastest is kept.
+  // Unless the caller set a second group, each smaller shape is then tried as
+  // one for the rods that find no room beside longer ones, and kept if faster.
   int fused_autotune(erod_batch* b, int precision, int trial_steps, double* best_ms) {
     int rc = fused_prepare(b, precision);
     if (rc) return rc;
@@ -319,17 +385,22 @@ namespace erod {
     struct Guard {
       erod_batch* b;
       double* snap[5] = {};
-      int lanes, epl;
+      int lanes, epl, split_lanes, split_epl;
       bool budgets;
+      bool keep_split = false;  // a second group chosen by the trials stays
       ~Guard() {
         for (double* p : snap)
           if (p) (void)hipFree(p);
         b->plan.lanes_per_tile = lanes;
         b->plan.elements_per_thread = epl;
+        if (!keep_split) {
+          b->plan.split_lanes = split_lanes;
+          b->plan.split_epl = split_epl;
+        }
         b->budgets = budgets;
         b->last_launches = 0;  // the trial launches are not a fused call's timing
       }
-    } g{b, {}, b->plan.lanes_per_tile, b->plan.elements_per_thread, b->budgets};
+    } g{b, {}, b->plan.lanes_per_tile, b->plan.elements_per_thread, b->plan.split_lanes, b->plan.split_epl, b->budgets};
     size_t bytes[5];
     for (int i = 0; i < 4; ++i) bytes[i] = sizeof(double) * field_info(snap_fields[i]).ncomp * b->field_slots(snap_fields[i]);
     bytes[4] = sizeof(double) * b->n_rods;
@@ -361,9 +432,33 @@ namespace erod {
       }
       if ((rc = restore())) break;
     }
+    // a second group of smaller tiles, when the caller has not set one
+    int best_split = -1;
+    if (rc == EROD_OK && best >= 0 && g.split_lanes == 0) {
+      const int cap = fused_shape_lanes(best);
+      for (int s2 = 0; s2 < fused::kNShapes && rc == EROD_OK; ++s2) {
+        if (fused_shape_lanes(s2) >= cap) continue;
+        b->plan.split_lanes = fused_shape_lanes(s2);
+        b->plan.split_epl = fused_shape_epl(s2);
+        if (plan_tiles(b, best) != EROD_OK || b->plan.shape2 < 0) continue;  // infeasible here, or no second group
+        if ((rc = fused_launch(b, precision, trial_steps, false))) break;  // warm
+        if ((rc = fused_launch(b, precision, trial_steps, true))) break;
+        float ms = 0.0f;
+        if ((rc = hip_check(hipEventSynchronize(b->ev1), "autotune sync"))) break;
+        if ((rc = hip_check(hipEventElapsedTime(&ms, b->ev0, b->ev1), "autotune time"))) break;
+        if (ms < best_t) {
+          best_split = s2;
+          best_t = ms;
+        }
+        if ((rc = restore())) break;
+      }
+    }
     // the chosen plan is made with the caller's settings and budgets
     b->plan.lanes_per_tile = g.lanes;
     b->plan.elements_per_thread = g.epl;
+    b->plan.split_lanes = best_split >= 0 ? fused_shape_lanes(best_split) : g.split_lanes;
+    b->plan.split_epl = best_split >= 0 ? fused_shape_epl(best_split) : g.split_epl;
+    g.keep_split = true;
     b->budgets = g.budgets;
     if (rc == EROD_OK && best >= 0) rc = plan_tiles(b, best);
     if (rc == EROD_OK) rc = restore();

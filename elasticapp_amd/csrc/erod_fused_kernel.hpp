@@ -965,26 +965,34 @@ namespace erod {
       }
     }
 
+    // the tiles of one group of a plan
+    struct Group {
+      uint64_t n_tiles;
+      const Tile* tiles;
+      const uint32_t* rods;
+      const uint32_t* lane0;
+    };
+
     template <typename R, int P, int NT, int kEnv>
-    inline void launch_env(erod_batch* b, const Consts<R>& C, int steps) {
+    inline void launch_env(erod_batch* b, const Consts<R>& C, int steps, const Group& g) {
       uint32_t* budgets = b->budgets ? b->d_steps_left : nullptr;
-      hipLaunchKernelGGL((k_fused<R, P, NT, kEnv>), dim3(static_cast<unsigned>(b->plan.n_tiles)), dim3(NT), 0, b->stream,
+      hipLaunchKernelGGL((k_fused<R, P, NT, kEnv>), dim3(static_cast<unsigned>(g.n_tiles)), dim3(NT), 0, b->stream,
                          b->d_field[EROD_POSITION], b->d_field[EROD_VELOCITY], b->d_field[EROD_DIRECTOR],
-                         b->d_field[EROD_ANGULAR_VELOCITY], C, b->d_rods, b->plan.d_tiles, b->plan.d_tile_rods,
-                         b->plan.d_tile_lane0, b->d_time, budgets, b->env, steps, b->E);
+                         b->d_field[EROD_ANGULAR_VELOCITY], C, b->d_rods, g.tiles, g.rods, g.lane0, b->d_time, budgets,
+                         b->env, steps, b->E);
     }
 
     // batches without gravity or contact run a kernel without environment code
     // (and without the registers its parameters would hold); a level contact
     // plane runs one without the arithmetic of a general normal
     template <typename R, int P, int NT>
-    inline int launch_tiles(erod_batch* b, const Consts<R>& C, int steps) {
-      if (b->plan.n_tiles == 0) return EROD_OK;
+    inline int launch_tiles(erod_batch* b, const Consts<R>& C, int steps, const Group& g) {
+      if (g.n_tiles == 0) return EROD_OK;
       const Env& e = b->env;
       const bool level = e.n[0] == 0.0 && e.n[1] == 0.0 && e.n[2] == 1.0;
-      if (e.contact && !level) launch_env<R, P, NT, 2>(b, C, steps);
-      else if (e.gravity || e.contact) launch_env<R, P, NT, 1>(b, C, steps);
-      else launch_env<R, P, NT, 0>(b, C, steps);
+      if (e.contact && !level) launch_env<R, P, NT, 2>(b, C, steps, g);
+      else if (e.gravity || e.contact) launch_env<R, P, NT, 1>(b, C, steps, g);
+      else launch_env<R, P, NT, 0>(b, C, steps, g);
       return hip_check(hipGetLastError(), "k_fused");
     }
 
@@ -995,20 +1003,32 @@ namespace erod {
     constexpr Shape kShapes[] = {{1, 64}, {1, 128}, {1, 256}, {1, 512}, {2, 256}, {2, 512}, {4, 128}, {4, 256}};
     constexpr int kNShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
-    // one launch of `steps` steps over every tile with the planned shape
     template <typename R>
-    inline int launch_all(erod_batch* b, const Consts<R>& C, int steps) {
-      switch (b->plan.shape) {
-        case 0: return launch_tiles<R, 1, 64>(b, C, steps);
-        case 1: return launch_tiles<R, 1, 128>(b, C, steps);
-        case 2: return launch_tiles<R, 1, 256>(b, C, steps);
-        case 3: return launch_tiles<R, 1, 512>(b, C, steps);
-        case 4: return launch_tiles<R, 2, 256>(b, C, steps);
-        case 5: return launch_tiles<R, 2, 512>(b, C, steps);
-        case 6: return launch_tiles<R, 4, 128>(b, C, steps);
-        case 7: return launch_tiles<R, 4, 256>(b, C, steps);
+    inline int launch_group(erod_batch* b, const Consts<R>& C, int steps, int shape, const Group& g) {
+      switch (shape) {
+        case 0: return launch_tiles<R, 1, 64>(b, C, steps, g);
+        case 1: return launch_tiles<R, 1, 128>(b, C, steps, g);
+        case 2: return launch_tiles<R, 1, 256>(b, C, steps, g);
+        case 3: return launch_tiles<R, 1, 512>(b, C, steps, g);
+        case 4: return launch_tiles<R, 2, 256>(b, C, steps, g);
+        case 5: return launch_tiles<R, 2, 512>(b, C, steps, g);
+        case 6: return launch_tiles<R, 4, 128>(b, C, steps, g);
+        case 7: return launch_tiles<R, 4, 256>(b, C, steps, g);
         default: set_error("bad tile shape"); return EROD_ERR_INVALID;
       }
+    }
+
+    // one launch of `steps` steps over every tile of the plan: its main group,
+    // then its second group when it has one (rods never interact, so the
+    // order does not matter)
+    template <typename R>
+    inline int launch_all(erod_batch* b, const Consts<R>& C, int steps) {
+      const FusedPlan& pl = b->plan;
+      if (int rc = launch_group(b, C, steps, pl.shape, Group{pl.n_tiles, pl.d_tiles, pl.d_tile_rods, pl.d_tile_lane0}))
+        return rc;
+      if (pl.shape2 >= 0 && pl.n_tiles2 > 0)
+        return launch_group(b, C, steps, pl.shape2, Group{pl.n_tiles2, pl.d_tiles2, pl.d_tile_rods2, pl.d_tile_lane02});
+      return EROD_OK;
     }
 
   }  // namespace fused

@@ -157,6 +157,15 @@ int erod_set_rod_time(erod_batch* b, const double* time_per_rod);
  * (one element per thread up to 256 lanes, two from 512). */
 int erod_set_fused_config(erod_batch* b, int lanes_per_tile, int elements_per_thread, int steps_per_launch);
 int erod_get_fused_config(const erod_batch* b, int* lanes_per_tile, int* elements_per_thread, int* steps_per_launch);
+/* A second group of tiles: a rod of at most lanes_per_tile elements that finds
+ * no room beside longer rods in the main tiles runs in tiles of this smaller
+ * shape (elements_per_thread 0 = either), launched after the main ones; 0, 0
+ * turns it off.  For fused FP64 on ragged batches, whose long-rod tiles spill
+ * registers while shorter tiles do not.  No counterpart in the reference (its
+ * blocks are sized by one policy, BlockingPolicy.hpp:291-321).  The getter
+ * reports the group's shape and tile count of the current plan (0 if none). */
+int erod_set_fused_split(erod_batch* b, int lanes_per_tile, int elements_per_thread);
+int erod_get_fused_split(const erod_batch* b, int* lanes_per_tile, int* elements_per_thread, uint64_t* tiles);
 /* Times every feasible tile shape on `trial_steps` steps from a snapshot of
  * the state (restored afterwards) and keeps the fastest; reports its ms/step.
  * No counterpart in the reference (its blocks are sized by a fixed policy,

@@ -432,3 +432,56 @@ def test_batch_length_pins_two_ghost_elements_per_gap():
     assert blk.element_slots == 46 and blk.starts == [0, 12, 24, 36]
     assert blk.get_field("Position").shape == (3, 47) and blk.get_field("ReferenceCurvature").shape == (3, 45)
     blk.close()
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32_BULK])
+def test_split_plan_equals_single_plan(precision):
+    """Rods that find no room beside longer rods in 1024-lane tiles run in a
+    second group of 512-lane tiles, launched after the first: every rod steps
+    bitwise as in the single plan (the arithmetic does not depend on the tile).
+    A split shape with more elements per thread than the shortest rod there
+    is refused."""
+    wl = random_workload(0, 0, 0, seed=37, steps=60, lengths=_edge_lengths())
+    a, b = device_block(wl), device_block(wl)
+    for blk in (a, b):
+        blk.set_fused_config(1024, 2, 30)
+    b.set_fused_split(512, 2)
+    a.step_fused(wl.steps, precision)
+    b.step_fused(wl.steps, precision)
+    lanes, epl, tiles = b.fused_split()
+    assert (lanes, epl) == (512, 2) and tiles > 0 and a.fused_split() == (0, 0, 0)
+    for r in range(len(wl.n_elems)):
+        for t in STATE:
+            assert np.array_equal(a.get_rod(t, r), b.get_rod(t, r)), (r, t)
+    assert np.array_equal(a.rod_time(), b.rod_time())
+    b.step_fused_graph(60, 30, precision)
+    a.step_fused_graph(60, 30, precision)
+    for r in range(0, len(wl.n_elems), 7):
+        assert np.array_equal(a.get_rod("Position", r), b.get_rod("Position", r)), r
+    b.set_fused_split(512, 4)
+    with pytest.raises(ErodError):
+        b.step_fused(1, precision)
+    for blk in (a, b):
+        blk.close()
+
+
+def test_autotune_keeps_a_second_group_only_if_faster():
+    """On a batch of short rods no second group can help (every rod fits the
+    smallest tiles' room), and the autotuner's plan steps bitwise like the
+    default plan; a second group set by the caller is kept as set."""
+    wl = random_workload(40, 4, 60, seed=43, steps=40)
+    a, b = device_block(wl), device_block(wl)
+    a.autotune(FP64, 10)
+    assert a.fused_split()[2] == 0 or a.fused_split()[0] < a.fused_config()[0]
+    a.step_fused(wl.steps, FP64)
+    b.step_fused(wl.steps, FP64)
+    for r in range(len(wl.n_elems)):
+        for t in STATE:
+            assert np.array_equal(a.get_rod(t, r), b.get_rod(t, r)), (r, t)
+    c = device_block(wl)
+    c.set_fused_config(256, 1, 0)
+    c.set_fused_split(128, 1)
+    c.autotune(FP64, 10)
+    assert c.fused_split()[:2] in ((128, 1), (0, 0))
+    for blk in (a, b, c):
+        blk.close()
