@@ -131,6 +131,15 @@ namespace erod {
     // whole steps on the fused persistent kernels
     void set_dt(double dt) { check(erod_set_rod_dt(b_, std::vector<double>(n_.size(), dt).data())); }
     void step_fused(std::uint64_t n, erod_precision p = EROD_FP64) { check(erod_step_fused(b_, p, n)); }
+    // tile shape chosen by measurement (and a second group of smaller tiles when faster)
+    double autotune(erod_precision p = EROD_FP64, int trial_steps = 100) {
+      double ms = 0.0;
+      check(erod_autotune_fused(b_, p, trial_steps, &ms));
+      return ms;
+    }
+    void set_split(int lanes_per_tile, int elements_per_thread) {
+      check(erod_set_fused_split(b_, lanes_per_tile, elements_per_thread));
+    }
     void synchronize() { check(erod_synchronize(b_)); }
 
     // slots and budgets (rods of different lengths and horizons in one batch)
