@@ -93,3 +93,17 @@ def test_bad_arguments_fail_before_any_device_call(lib):
     done = ctypes.c_uint64()
     assert L.erod_server_tick(None, None, 0, ctypes.byref(done)) != 0 and b"null server" in L.erod_last_error()
     assert L.erod_load(b"/nonexistent/erod.ckpt", ctypes.byref(out)) != 0 and b"cannot open" in L.erod_last_error()
+
+
+def test_header_is_plain_c(tmp_path):
+    """include/erod.h compiles as C99 with -pedantic: a cgo or FFI binding can
+    include it as it is."""
+    import shutil
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "uses_header.c"
+    src.write_text('#include "erod.h"\nint main(void) { return erod_batch_destroy((erod_batch*)0); }\n')
+    r = subprocess.run([cc, "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-fsyntax-only",
+                        "-I", os.path.join(ROOT, "include"), str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
