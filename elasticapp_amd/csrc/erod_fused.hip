@@ -440,7 +440,12 @@ namespace erod {
         if (fused_shape_lanes(s2) >= cap) continue;
         b->plan.split_lanes = fused_shape_lanes(s2);
         b->plan.split_epl = fused_shape_epl(s2);
-        if (plan_tiles(b, best) != EROD_OK || b->plan.shape2 < 0) continue;  // infeasible here, or no second group
+        const int prc = plan_tiles(b, best);
+        if (prc == EROD_ERR_HIP) {  // a device failure ends the search; an infeasible shape does not
+          rc = prc;
+          break;
+        }
+        if (prc != EROD_OK || b->plan.shape2 < 0) continue;  // infeasible here, or no second group
         if ((rc = fused_launch(b, precision, trial_steps, false))) break;  // warm
         if ((rc = fused_launch(b, precision, trial_steps, true))) break;
         float ms = 0.0f;
