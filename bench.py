@@ -56,22 +56,45 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_fp32_ragged4096_pmc.md")
 CONTACT_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_fp32_rods_on_plane_pmc.md")
 
 
-def measured_traffic(lanes, epl, spl, tag="traffic", path=TRAFFIC_FILE):
-    """HBM bytes per launch of the FP32-bulk kernel at this tile shape, from
-    the rocprofv3 FETCH_SIZE / WRITE_SIZE summary (lines of the form
-    `traffic lanes=L epl=P spl=S bytes_per_launch=B` for the ragged batch,
-    `traffic_contact ...` for the rods on a plane), or None."""
+def _summary_lines(path, tag, nfields):
+    """(lanes, epl, spl, key=value dict) of the machine-readable lines `tag
+    lanes=L epl=P spl=S ...` of a counter summary under profiles/."""
+    out = []
     try:
         with open(path) as f:
             for line in f:
                 w = line.strip().strip("`").split()
-                if len(w) == 5 and w[0] == tag:
+                if len(w) == nfields and w[0] == tag:
                     kv = dict(x.split("=", 1) for x in w[1:])
-                    if (int(kv["lanes"]), int(kv["epl"]), int(kv["spl"])) == (lanes, epl, spl):
-                        return float(kv["bytes_per_launch"])
+                    out.append((int(kv["lanes"]), int(kv["epl"]), int(kv["spl"]), kv))
     except (OSError, ValueError, KeyError):
         pass
-    return None
+    return out
+
+
+def _closest(lines, lanes, epl, spl):
+    """The line of this tile shape measured at `spl` steps per launch, else at
+    the nearest step count; None without one for the shape."""
+    cand = [(abs(s - spl), s, kv) for (l, e, s, kv) in lines if (l, e) == (lanes, epl)]
+    if not cand:
+        return None
+    cand.sort(key=lambda c: c[0])
+    return cand[0][1], cand[0][2]
+
+
+def measured_traffic(lanes, epl, spl, tag="traffic", path=TRAFFIC_FILE, exact=True):
+    """HBM bytes per launch of the FP32-bulk kernel at this tile shape, from
+    the rocprofv3 FETCH_SIZE / WRITE_SIZE summary (lines of the form
+    `traffic lanes=L epl=P spl=S bytes_per_launch=B` for the ragged batch,
+    `traffic_contact ...` for the rods on a plane), or None.  With
+    exact=False the line of the nearest steps per launch is taken (one pass
+    over state and constants per launch whatever its steps) and the result is
+    (bytes, spl measured at)."""
+    c = _closest(_summary_lines(path, tag, 5), lanes, epl, spl)
+    if c is None or (exact and c[0] != spl):
+        return None
+    b = float(c[1]["bytes_per_launch"])
+    return b if exact else (b, c[0])
 
 
 # VALU issue peak of MI355X: 256 CUs x 2.4 GHz, each CU issuing one wave64
@@ -81,22 +104,17 @@ def measured_traffic(lanes, epl, spl, tag="traffic", path=TRAFFIC_FILE):
 VALU_PEAK_GINSTR = 256 * 2.4
 
 
-def measured_valu(lanes, epl, spl):
+def measured_valu(lanes, epl, spl, exact=True):
     """(waves per launch, VALU instructions per wave and step) of the
     FP32-bulk kernel at this tile shape on the ragged batch, from the
     rocprofv3 SQ counter summary (`valu lanes=L epl=P spl=S waves=W
-    insts_per_wave_step=I`), or None."""
-    try:
-        with open(TRAFFIC_FILE) as f:
-            for line in f:
-                w = line.strip().strip("`").split()
-                if len(w) == 6 and w[0] == "valu":
-                    kv = dict(x.split("=", 1) for x in w[1:])
-                    if (int(kv["lanes"]), int(kv["epl"]), int(kv["spl"])) == (lanes, epl, spl):
-                        return int(kv["waves"]), float(kv["insts_per_wave_step"])
-    except (OSError, ValueError, KeyError):
-        pass
-    return None
+    insts_per_wave_step=I`), or None.  With exact=False the line of the
+    nearest steps per launch is taken and its step count returned third."""
+    c = _closest(_summary_lines(TRAFFIC_FILE, "valu", 6), lanes, epl, spl)
+    if c is None or (exact and c[0] != spl):
+        return None
+    r = (int(c[1]["waves"]), float(c[1]["insts_per_wave_step"]))
+    return r if exact else r + (c[0],)
 
 
 def stream_copy_gbs(torch, nbytes=2 << 30, reps=10):
@@ -125,15 +143,30 @@ def timed(fn, sync):
     return time.perf_counter() - t0
 
 
-def cpu_baseline(wl, budget_s=12.0):
+def cpu_model_name():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
+
+
+def cpu_baseline(wl, budget_s=12.0, n_sample=512):
     """The oracle (FP64 CPU restatement) on a bounded sample of the same batch:
-    the first rods of the ragged batch packed into blocks of <= 1024 elements,
-    one std::thread per host core, timed over enough steps to fill ~budget_s."""
+    every k-th rod of the ragged batch (n_sample rods spread over the whole
+    length distribution) packed into blocks of <= 1024 elements, one
+    std::thread per host core of one GPU's share, timed over enough steps to
+    fill ~budget_s."""
     from elasticapp_amd.workloads import Workload
     from tests.gpu_helpers import oracle_system
     threads = min(16, len(os.sched_getaffinity(0)))  # the host share of one GPU
-    n_rods = min(512, len(wl.n_elems))
-    sub = Workload("cpu_sample", wl.n_elems[:n_rods], wl.states[:n_rods], wl.dt, 0, forces=wl.forces[:n_rods])
+    stride = max(1, len(wl.n_elems) // n_sample)
+    idx = list(range(0, len(wl.n_elems), stride))[:n_sample]
+    pos = {r: k for k, r in enumerate(idx)}  # end forces follow their rods into the sample's numbering
+    sub = Workload("cpu_sample", [wl.n_elems[i] for i in idx], [wl.states[i] for i in idx], wl.dt, 0,
+                   forces=[(pos[f[0]],) + tuple(f[1:]) for f in wl.forces if f[0] in pos])
     sysm = oracle_system(sub, max_block_elems=1024)
     elems = sum(sub.n_elems)
     t = time.perf_counter()
@@ -143,17 +176,12 @@ def cpu_baseline(wl, budget_s=12.0):
     t = time.perf_counter()
     sysm.step(wl.dt, steps, threads)
     el = time.perf_counter() - t
-    cpu_model = ""
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                cpu_model = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
     return {"value": elems * steps / el, "unit": "element-steps/s", "cores": threads, "kind": "port",
-            "sample": "first %d rods of the batch (%d elements) in blocks of <=1024 elements, %d FP64 steps, "
-                      "%.1f s, %s" % (n_rods, elems, steps, el, cpu_model)}
+            "sample": "every %dth rod of the batch (%d rods, %d elements, %.0f%% of its elements) in blocks of "
+                      "<=1024 elements, %d FP64 steps, %.1f s on %d threads of %s (oracle built -O3 "
+                      "-march=x86-64-v3 -ffp-contract=off)"
+                      % (stride, len(idx), elems, 100.0 * elems / sum(wl.n_elems), steps, el, threads,
+                         cpu_model_name())}
 
 
 def contact_leg(args, torch):
@@ -235,7 +263,7 @@ def small_batches_leg(args, torch, cpu):
     batch (1024 rods x 64, 10,000 steps): FP64 fused and plain rates, the
     fused FP64 error against the oracle for the cantilever, the roofline
     fraction of the uniform batch, and the oracle timed beside them."""
-    from elasticapp_amd import FP64
+    from elasticapp_amd import FP32_BULK, FP64
     from elasticapp_amd.workloads import build_block, timoshenko_cantilever, uniform_1024x64
     from tests.gpu_helpers import oracle_system, rel_state_error
     threads = min(16, len(os.sched_getaffinity(0)))
@@ -266,18 +294,33 @@ def small_batches_leg(args, torch, cpu):
     blk.set_fused_config(0, 0, 100)
     blk.autotune(FP64, 100)
     sync = lambda: (blk.synchronize(), torch.cuda.synchronize())
-    blk.step_fused_graph(100, 100, FP64)
-    sync()
     el = timed(lambda: blk.step_fused_graph(wl.steps, 100, FP64), sync)
     kms, launches = blk.last_kernel_ms()
-    achieved = float(b64(n).sum()) * (wl.steps // launches) / (kms * 1e-3) / 1e9
+    eq = float(b64(n).sum()) * (wl.steps // launches) / (kms * 1e-3) / 1e9
     elems = int(n.sum())
-    out["uniform_1024x64"] = {"steps": wl.steps, "fp64_fused": elems * wl.steps / el, "unit": "element-steps/s",
-                         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                      "frac": achieved / HBM_PEAK_GBS, "kernel_ms_per_launch": kms,
-                                      "algorithmic_bytes_per_launch": float(b64(n).sum()) * (wl.steps // launches)},
-                         "cpu_baseline": None if not cpu else _cpu_rate(wl, 1024, 5.0, threads)}
-    log("1024 x 64 uniform rods: fp64 fused %.3e element-steps/s, %.0f GB/s algorithmic" % (out["uniform_1024x64"]["fp64_fused"], achieved))
+    # FP32-bulk over the same steps: its rate and its position error against FP64
+    b32_ = build_block(wl)
+    b32_.set_fused_config(0, 0, 100)
+    b32_.autotune(FP32_BULK, 100)  # trial launches from a snapshot, restored afterwards
+    sync32 = lambda: (b32_.synchronize(), torch.cuda.synchronize())
+    el32 = timed(lambda: b32_.step_fused_graph(wl.steps, 100, FP32_BULK), sync32)
+    x32, x64 = b32_.get_field("Position"), blk.get_field("Position")
+    L = np.array([st["ReferenceElementLength"].sum() for st in wl.states])
+    err32 = max(float(np.abs(x32[:, s0:s0 + k + 1] - x64[:, s0:s0 + k + 1]).max() / L[i])
+                for i, (s0, k) in enumerate(zip(blk.starts, n)))
+    b32_.close()
+    blk.close()
+    out["uniform_1024x64"] = {"steps": wl.steps, "fp64_fused": elems * wl.steps / el,
+                              "fp32_bulk": elems * wl.steps / el32, "unit": "element-steps/s",
+                              "fp32_bulk_position_err_vs_fp64": err32,
+                              "fp64_fused_kernel_ms_per_launch": kms, "fp64_fused_launches": launches,
+                              "fp64_equivalent_streaming_gbs": eq,
+                              "note": "the fused FP64 launch holds the state on chip for 100 steps, so its "
+                                      "equivalent streaming bytes (432 n + 48 per rod-step) over the launch time are "
+                                      "no HBM fraction",
+                              "cpu_baseline": None if not cpu else _cpu_rate(wl, 1024, 5.0, threads)}
+    log("1024 x 64 uniform rods: fp64 fused %.3e, fp32-bulk %.3e element-steps/s, fp32-bulk position error %.2e"
+        % (out["uniform_1024x64"]["fp64_fused"], out["uniform_1024x64"]["fp32_bulk"], err32))
     return out
 
 
@@ -285,9 +328,11 @@ def serving_leg(args, torch, capacity):
     """Continuous batching: Poisson arrivals of requests (1-64 template rods of
     32-512 elements, horizons of 1e3-1e5 steps) served from a pool of rod slots
     by the FP32-bulk fused kernel; latency = completion - arrival.  The arrival
-    rate is 70% of the server's own capacity, measured first as the
-    element-steps per second it advances with a backlog that keeps its queue
-    non-empty (`capacity` is the batched rate of the ragged batch, for
+    rate is fixed (`--serve-rate`, requests per second), so that p50 / p99 are
+    comparable between runs and builds; the server's own capacity, measured
+    first as the element-steps per second it advances with a backlog that
+    keeps its queue non-empty, is reported beside it with the load the rate
+    offers (`capacity` is the batched rate of the ragged batch, for
     comparison)."""
     from elasticapp_amd import FP32_BULK, Server
     from elasticapp_amd.workloads import build_block, serving_library, serving_requests
@@ -318,7 +363,7 @@ def serving_leg(args, torch, capacity):
     server_capacity = (e1 - e0) / sat_wall
     sat.close()
     log("serving, server capacity: %.3e element-steps/s over %d backlogged ticks" % (server_capacity, k1 - k0))
-    lam = 0.7 * server_capacity / mean_work
+    lam = args.serve_rate
     rng = np.random.default_rng(3005)
     arrivals = np.cumsum(rng.exponential(1.0 / lam, len(reqs)))
     srv = Server(lib, classes, precision=FP32_BULK, steps_per_tick=spt, hbm_budget_bytes=200e9)
@@ -348,7 +393,7 @@ def serving_leg(args, torch, capacity):
     thr = float(work.sum()) / wall
     out = {"workload": "%d requests, U{1..64} rods of %d templates (n logU[32,512], materials as the ragged batch, "
                        "own stable dt), horizons logU[1e3,1e5] steps, Poisson arrivals" % (len(reqs), len(n_lib)),
-           "lambda_per_s": lam, "target_utilization": 0.7, "server_capacity_element_steps_per_s": server_capacity,
+           "lambda_per_s": lam, "lambda": "fixed", "server_capacity_element_steps_per_s": server_capacity,
            "server_capacity_ticks": k1 - k0, "ragged_batch_element_steps_per_s": capacity,
            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
            "mean_s": float(np.mean(lat)), "max_s": float(np.max(lat)), "throughput": thr, "unit": "element-steps/s",
@@ -362,18 +407,29 @@ def serving_leg(args, torch, capacity):
     return out
 
 
+def run_steps(blk, n, spl, precision):
+    """n fused steps of `spl` steps per launch: whole hipGraph chunks of spl
+    steps, the rest (n % spl, fewer than one chunk) as one direct launch."""
+    if n > 0:
+        blk.step_fused_graph(n, spl, precision)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10000)
-    ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--chunk", type=int, default=100, help="steps per captured hipGraph replay")
+    ap.add_argument("--steps", type=int, default=10000, help="timed steps of the ragged batch")
+    ap.add_argument("--warmup", type=int, default=100, help="untimed steps before them")
+    ap.add_argument("--chunk", type=int, default=100,
+                    help="steps per launch and per captured hipGraph replay (at most --steps)")
+    ap.add_argument("--horizon", type=int, default=10000,
+                    help="steps after which the FP32-bulk positions are compared with FP64 (at least warmup + steps)")
     ap.add_argument("--plain-steps", type=int, default=200, help="steps timed on the op-by-op FP64 path")
     ap.add_argument("--stream-steps", type=int, default=200, help="steps timed at one step per launch")
     ap.add_argument("--contact-steps", type=int, default=2000, help="steps of the contact batch (0: skip it)")
     ap.add_argument("--contact-plain-steps", type=int, default=20)
     ap.add_argument("--serve-requests", type=int, default=2000, help="requests of the serving leg (0: skip it)")
     ap.add_argument("--serve-steps-per-tick", type=int, default=500, help="fused steps per server tick")
+    ap.add_argument("--serve-rate", type=float, default=450.0, help="Poisson arrival rate of the serving leg (1/s)")
     ap.add_argument("--small-batches", type=int, default=1, help="1: also run the cantilever and the uniform batch")
     ap.add_argument("--tile", default="", help="LANES,EPL: fixed tile shape instead of the autotuner's choice")
     ap.add_argument("--no-cpu", action="store_true")
@@ -383,9 +439,13 @@ def main():
         print("bench.py runs on one GPU only: liberod has no multi-GPU path (rods are independent, so a batch "
               "would shard by rods with no exchange per step)", file=sys.stderr)
         sys.exit(2)
-    if args.steps % args.chunk or args.warmup % args.chunk or args.contact_steps % args.chunk:
-        print("--steps, --warmup and --contact-steps must be multiples of --chunk", file=sys.stderr)
+    if args.steps < 1 or args.warmup < 0 or args.chunk < 1 or args.contact_steps < 0:
+        print("--steps and --chunk must be positive, --warmup and --contact-steps not negative", file=sys.stderr)
         sys.exit(2)
+    # steps per launch of the timed region: a launch holds the rod state on
+    # chip for its steps, so the timed steps are whole launches of at most
+    # --chunk steps (fewer timed steps than a chunk: one launch of all of them)
+    spl = min(args.chunk, args.steps)
 
     import torch
     from elasticapp_amd import FP32_BULK, FP64
@@ -397,25 +457,26 @@ def main():
     wl = ragged_4096(n_rods=args.rods, steps=args.steps)
     n = np.array(wl.n_elems)
     elems = int(n.sum())
-    log("workload ragged_4096: %d rods, %d elements, dt per rod %.3e .. %.3e" % (
-        len(n), elems, float(np.min(wl.info["rod_dt"])), float(np.max(wl.info["rod_dt"]))))
+    log("workload ragged_4096: %d rods, %d elements, dt per rod %.3e .. %.3e; %d timed steps after %d warm-up steps, "
+        "%d steps per launch" % (len(n), elems, float(np.min(wl.info["rod_dt"])), float(np.max(wl.info["rod_dt"])),
+                                 args.steps, args.warmup, spl))
 
     copy_gbs = stream_copy_gbs(torch)
     log("device-to-device copy: %.0f GB/s" % copy_gbs)
 
-    # ---- FP32-bulk, hipGraph chunks ----------------------------------------------------
+    # ---- FP32-bulk: the timed steps ------------------------------------------------------
     # every rod steps with its own stable dt (the axial and rotational limits)
     rod_dt = np.asarray(wl.info["rod_dt"])
     blk32 = make_block(wl)
     blk32.set_rod_dt(rod_dt)
     if args.tile:
         lanes, epl = (int(x) for x in args.tile.split(","))
-        blk32.set_fused_config(lanes, epl, args.chunk)
+        blk32.set_fused_config(lanes, epl, spl)
         log("fixed shape (lanes, elements/thread, steps/launch) = %s" % (blk32.fused_config(),))
     else:
-        blk32.set_fused_config(0, 0, args.chunk)
-        # one trial launch of `chunk` steps per tile shape
-        tune_ms = blk32.autotune(FP32_BULK, args.chunk)
+        blk32.set_fused_config(0, 0, spl)
+        # one trial launch of `spl` steps per tile shape
+        tune_ms = blk32.autotune(FP32_BULK, spl)
         log("autotuned shape (lanes, elements/thread, steps/launch) = %s, second group %s, %.4f ms/step"
             % (blk32.fused_config(), blk32.fused_split(), tune_ms))
 
@@ -423,49 +484,68 @@ def main():
         blk32.synchronize()
         torch.cuda.synchronize()
 
-    blk32.step_fused_graph(args.warmup, args.chunk, FP32_BULK)
+    run_steps(blk32, args.warmup, spl, FP32_BULK)
     sync32()
-    el32 = timed(lambda: blk32.step_fused_graph(args.steps, args.chunk, FP32_BULK), sync32)
+    el32 = timed(lambda: run_steps(blk32, args.steps, spl, FP32_BULK), sync32)
     v32 = elems * args.steps / el32
-    # dominant kernel: HIP events on the batch stream around the timed replays
+    # dominant kernel: HIP events on the batch stream around the timed launches
     kms, launches = blk32.last_kernel_ms()
-    log("fp32-bulk: %.3f s for %d steps -> %.3e element-steps/s; %d launches, %.4f ms/launch"
+    log("fp32-bulk: %.4f s for %d steps -> %.3e element-steps/s; %d launches, %.4f ms/launch"
         % (el32, args.steps, v32, launches, kms))
-    flags = blk32.check_finite()
-    assert not flags.any(), "non-finite FP32-bulk state in %d rods" % int(flags.sum())
+    lanes, epl, _ = blk32.fused_config()
+    split32 = blk32.fused_split()
 
-    lanes, epl, spl = blk32.fused_config()
-    steps_per_launch = args.steps // launches
-    bytes_launch = float(b32(n).sum()) * steps_per_launch
-    achieved = bytes_launch / (kms * 1e-3) / 1e9
-
-    # ---- fused FP64 over the same steps (error reference and FP64 rate) ----------------
+    # ---- fused FP64 over the same steps ----------------------------------------------------
     blk64 = make_block(wl)
     blk64.set_rod_dt(rod_dt)
-    blk64.set_fused_config(0, 0, args.chunk)
-    blk64.autotune(FP64, args.chunk)
+    blk64.set_fused_config(0, 0, spl)
+    blk64.autotune(FP64, spl)
 
     def sync64():
         blk64.synchronize()
         torch.cuda.synchronize()
 
-    blk64.step_fused_graph(args.warmup, args.chunk, FP64)
+    run_steps(blk64, args.warmup, spl, FP64)
     sync64()
-    el64 = timed(lambda: blk64.step_fused_graph(args.steps, args.chunk, FP64), sync64)
+    el64 = timed(lambda: run_steps(blk64, args.steps, spl, FP64), sync64)
     v64 = elems * args.steps / el64
     plan64 = {"tile": list(blk64.fused_config()), "second_group": list(blk64.fused_split())}
-    log("fp64 fused: %.3f s -> %.3e element-steps/s (tiles %s, second group %s)"
-        % (el64, v64, plan64["tile"], plan64["second_group"]))
+    log("fp64 fused: %.4f s for %d steps -> %.3e element-steps/s (tiles %s, second group %s)"
+        % (el64, args.steps, v64, plan64["tile"], plan64["second_group"]))
+
+    # ---- both on to the error horizon, in launches of --chunk steps (timed: the
+    # sustained rates of long runs) --------------------------------------------------------
+    horizon = max(args.horizon, args.warmup + args.steps)
+    rest = horizon - args.warmup - args.steps
+    sustained = None
+    if rest > 0:
+        for blk, prec in ((blk32, FP32_BULK), (blk64, FP64)):
+            lz, ez, _ = blk.fused_config()
+            blk.set_fused_config(lz, ez, args.chunk)
+        els32 = timed(lambda: run_steps(blk32, rest, args.chunk, FP32_BULK), sync32)
+        kms_s32, launches_s32 = blk32.last_kernel_ms()
+        els64 = timed(lambda: run_steps(blk64, rest, args.chunk, FP64), sync64)
+        sustained = {"steps": rest, "steps_per_launch": args.chunk,
+                     "fp32_bulk": elems * rest / els32, "fp64_fused": elems * rest / els64,
+                     "fp32_bulk_kernel_ms_per_launch": kms_s32, "fp32_bulk_launches": launches_s32,
+                     "unit": "element-steps/s",
+                     "note": "the steps from the timed ones up to the error horizon, in hipGraph chunks of "
+                             "steps_per_launch steps"}
+        log("sustained over %d more steps at %d steps/launch: fp32-bulk %.3e, fp64 fused %.3e element-steps/s"
+            % (rest, args.chunk, sustained["fp32_bulk"], sustained["fp64_fused"]))
+    flags = blk32.check_finite()
+    assert not flags.any(), "non-finite FP32-bulk state in %d rods" % int(flags.sum())
     # error of the FP32-bulk positions against FP64 after the same steps:
     # per rod max |x32 - x64| over its nodes, relative to the rod length
     x32 = blk32.get_field("Position")
     x64 = blk64.get_field("Position")
     lengths = np.array([st["ReferenceElementLength"].sum() for st in wl.states])
-    pos_err = 0.0
+    rod_err = np.empty(len(n))
     for i, s0 in enumerate(blk64.starts):
-        d = np.abs(x32[:, s0:s0 + n[i] + 1] - x64[:, s0:s0 + n[i] + 1]).max() / lengths[i]
-        pos_err = max(pos_err, float(d))
-    log("fp32-bulk position error vs fp64 after %d steps: max_rod |dx|/L = %.3e" % (args.warmup + args.steps, pos_err))
+        rod_err[i] = np.abs(x32[:, s0:s0 + n[i] + 1] - x64[:, s0:s0 + n[i] + 1]).max() / lengths[i]
+    pos_err = float(rod_err.max())
+    log("fp32-bulk position error vs fp64 after %d steps: max over rods |dx|/L = %.3e (median %.3e)"
+        % (horizon, pos_err, float(np.median(rod_err))))
     blk64.close()
 
     # the same kernel and tile shape at one step per launch: the state goes
@@ -479,18 +559,35 @@ def main():
     log("fp32-bulk at 1 step/launch: %.4f ms/launch -> %.0f GB/s algorithmic (%.3e element-steps/s)"
         % (kms_s, achieved_s, elems / (kms_s * 1e-3)))
     blk32.close()
-    traffic = measured_traffic(lanes, epl, spl)
-    traffic_s = measured_traffic(lanes, epl, 1)
-    valu = measured_valu(lanes, epl, spl)
-    roofline_valu = None
+
+    # ---- roofline of the timed launches ----------------------------------------------------
+    # With the state held on chip for the steps of a launch, HBM is touched
+    # once per launch (state in and out, constants in), so what bounds the
+    # launch is VALU issue: its VALU instructions (counter summary, per wave
+    # and step) over its event-timed duration, against one wave64 VALU
+    # instruction per CU per cycle.  The measured HBM traffic and the bytes a
+    # streaming kernel would move for the same steps are reported beside it.
+    bytes_alg = float(b32(n).sum()) * (args.steps / launches)
+    traffic = measured_traffic(lanes, epl, spl, exact=False)
+    valu = measured_valu(lanes, epl, spl, exact=False)
+    steps_launch = args.steps / launches
+    roofline = {"bound": "valu", "achieved": None, "peak": VALU_PEAK_GINSTR, "unit": "G wave64 VALU instr/s",
+                "frac": None, "traffic": traffic[0] if traffic else None,
+                "kernel": "k_fused<float,%d,%d> (FP32-bulk)" % (epl, lanes // epl),
+                "kernel_ms_per_launch": kms, "launches": launches, "steps_per_launch": steps_launch}
     if valu is not None:
-        waves, ipws = valu
-        g = waves * ipws * steps_per_launch / (kms * 1e-3) / 1e9
-        roofline_valu = {"bound": "valu", "achieved": g, "peak": VALU_PEAK_GINSTR, "unit": "G wave64 VALU instr/s",
-                         "frac": g / VALU_PEAK_GINSTR, "insts_per_wave_step": ipws, "waves_per_launch": waves,
-                         "note": "what bounds the kernel once its state is on chip: the VALU instruction count per "
-                                 "wave and step (rocprofv3 SQ_INSTS_VALU, committed summary) over the event-timed "
-                                 "launch, against one wave64 VALU instruction per CU per cycle"}
+        waves, ipws, valu_spl = valu
+        g = waves * ipws * steps_launch / (kms * 1e-3) / 1e9
+        roofline.update(achieved=g, frac=g / VALU_PEAK_GINSTR, insts_per_wave_step=ipws, waves_per_launch=waves,
+                        counters_at_steps_per_launch=valu_spl)
+    if traffic is not None:
+        hbm = traffic[0] / (kms * 1e-3) / 1e9
+        roofline["hbm_measured"] = {"achieved": hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm / HBM_PEAK_GBS,
+                                    "bytes_per_launch": traffic[0], "counters_at_steps_per_launch": traffic[1]}
+    roofline["equivalent_streaming"] = {
+        "bytes_per_launch": bytes_alg, "GB_per_s": bytes_alg / (kms * 1e-3) / 1e9, "unit": "GB/s",
+        "note": "B32(n) = 360 n + 72 bytes per rod-step times the launch's steps: what a kernel streaming the "
+                "state through HBM every step would move; not a fraction of peak (the state stays on chip)"}
 
     # ---- plain FP64 (one kernel per reference operator) ----------------------------------
     blkp = make_block(wl)
@@ -529,22 +626,18 @@ def main():
                    "rods": int(len(n)), "elements": elems,
                    "dt_per_rod": {"min": float(rod_dt.min()), "median": float(np.median(rod_dt)),
                                   "max": float(rod_dt.max())},
-                   "graph_chunk": args.chunk,
-                   "tile_lanes": lanes, "elements_per_thread": epl, "steps_per_launch": spl, "device": dev_name},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_fused<float,%d,%d> (FP32-bulk)" % (epl, lanes // epl),
-                     "kernel_ms_per_launch": kms, "launches": launches, "steps_per_launch": steps_per_launch,
-                     "algorithmic_bytes_per_launch": bytes_launch,
-                     "note": "state held on chip across the steps of a launch: HBM traffic per launch is "
-                             "`traffic`, far below the algorithmic bytes, so frac can exceed 1"},
+                   "steps_per_launch": spl, "graph_chunk": spl,
+                   "tile_lanes": lanes, "elements_per_thread": epl, "second_group": list(split32),
+                   "device": dev_name},
+        "roofline": roofline,
         "roofline_streaming": {"bound": "hbm", "achieved": achieved_s, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                               "frac": achieved_s / HBM_PEAK_GBS, "traffic": traffic_s,
+                               "frac": achieved_s / HBM_PEAK_GBS,
+                               "traffic": measured_traffic(lanes, epl, 1),
                                "kernel_ms_per_launch": kms_s, "launches": launches_s, "steps_per_launch": 1,
                                "algorithmic_bytes_per_launch": bytes_s,
                                "element_steps_per_s": elems / (kms_s * 1e-3)},
-        "roofline_valu": roofline_valu,
         "stream_copy_gbs": copy_gbs,
+        "sustained": sustained,
         "cantilever_and_uniform": small,
         "rods_on_plane": contact,
         "serving": serving,
@@ -554,8 +647,10 @@ def main():
                                 "second group (lanes, elements/thread, tiles)": plan64["second_group"]}},
         "fp64_plain": {"value": vp, "steps": args.plain_steps},
         "ratio_fp32bulk_vs_plain_fp64": v32 / vp,
-        "fp32_bulk_position_err_vs_fp64": {"value": pos_err, "metric": "max over rods of max |x32 - x64| / rod length",
-                                           "after_steps": args.warmup + args.steps},
+        "ratio_fp32bulk_vs_fused_fp64": v32 / v64,
+        "fp32_bulk_position_err_vs_fp64": {"value": pos_err, "median_over_rods": float(np.median(rod_err)),
+                                           "metric": "max over rods of max |x32 - x64| / rod length",
+                                           "after_steps": horizon},
     }
     print(json.dumps(out))
 

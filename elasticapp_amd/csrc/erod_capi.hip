@@ -694,14 +694,15 @@ int erod_step_fused(erod_batch* b, int precision, uint64_t n) {
 }
 
 int erod_step_fused_graph(erod_batch* b, int precision, uint64_t n, uint64_t chunk) {
-  if (check_batch(b) || chunk == 0 || n % chunk != 0) {
-    set_error("erod_step_fused_graph: n must be a positive multiple of chunk");
+  if (check_batch(b) || chunk == 0) {
+    set_error("erod_step_fused_graph: chunk must be positive");
     return EROD_ERR_INVALID;
   }
   int rc = fused_common(b, precision);
   if (rc) return rc;
   const uint64_t spl = b->plan.steps_per_launch > 0 ? uint64_t(b->plan.steps_per_launch) : 100;
-  if (b->graph_precision != precision || b->graph_chunk != chunk || !b->graph_exec) {
+  const uint64_t whole = n / chunk, rest = n % chunk;
+  if (whole > 0 && (b->graph_precision != precision || b->graph_chunk != chunk || !b->graph_exec)) {
     if (b->graph_exec) {
       hipGraphExecDestroy(b->graph_exec);
       b->graph_exec = nullptr;
@@ -732,12 +733,24 @@ int erod_step_fused_graph(erod_batch* b, int precision, uint64_t n, uint64_t chu
     b->graph_launches = launches;
   }
   EROD_HIP(hipEventRecord(b->ev0, b->stream));
-  for (uint64_t i = 0; i < n / chunk; ++i) {
+  for (uint64_t i = 0; i < whole; ++i) {
     EROD_HIP(hipGraphLaunch(b->graph_exec, b->stream));
     for (uint64_t d = 0; d < chunk; d += spl) spend_budgets(b, std::min<uint64_t>(spl, chunk - d));
   }
+  uint64_t launches = whole * b->graph_launches;
+  // the steps that do not fill a chunk: direct launches of the same length
+  // a chunk's launches have (spl, the last one shorter), so the launch
+  // segmentation equals that of erod_step_fused over n steps whenever chunk
+  // is a multiple of spl
+  for (uint64_t done = 0; done < rest;) {
+    const uint64_t s = std::min<uint64_t>(spl, rest - done);
+    if ((rc = fused_launch(b, precision, static_cast<int>(s), false))) return rc;
+    spend_budgets(b, s);
+    ++launches;
+    done += s;
+  }
   EROD_HIP(hipEventRecord(b->ev1, b->stream));
-  b->last_launches = (n / chunk) * b->graph_launches;
+  b->last_launches = launches;
   return EROD_OK;
 }
 

@@ -87,6 +87,7 @@ namespace erod {
     int elements_per_thread = 0;  // requested elements per thread (0 = any)
     int steps_per_launch = 0;     // requested (0 = 100)
     int shape = -1;               // chosen index into fused::kShapes
+    int tuned_shape = -1;         // the autotuner's choice (-1: none), preferred while feasible
     uint64_t n_tiles = 0;
     uint32_t* d_tile_rods = nullptr;  // rod ids in tile order
     uint32_t* d_tile_lane0 = nullptr; // first lane of each rod within its tile

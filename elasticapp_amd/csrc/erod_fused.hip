@@ -110,8 +110,11 @@ namespace erod {
   // two elements per thread, two 4-wave workgroups per CU, run 27% faster than
   // one element per thread in 8-wave workgroups; four elements per thread
   // spill and run one wave per SIMD.)
+  // A shape the autotuner chose is taken again while it stays feasible (a
+  // replan after rods changed length keeps it, with its second group).
   static int default_shape(const erod_batch* b) {
     static const int pref[] = {0, 1, 2, 4, 5, 3, 7, 6};
+    if (b->plan.tuned_shape >= 0 && shape_feasible(b, b->plan.tuned_shape)) return b->plan.tuned_shape;
     for (int si : pref)
       if (shape_feasible(b, si)) return si;
     return -1;
@@ -135,9 +138,11 @@ namespace erod {
   // a serving pool replans its ~30,000 slots in about 0.5 ms of host time.
   // Here `order` holds the rods, longest first, and cap is P * NT.  With
   // spill_cap > 0, a rod of at most spill_cap elements that finds no room in an
-  // open tile goes to `overflow` instead of opening one.
+  // open tile goes to `overflow` instead of opening one, provided it has at
+  // least spill_min elements (the second group's elements per thread; a
+  // shorter rod stays in the main group).
   static void pack(const erod_batch* b, uint64_t cap, const std::vector<uint32_t>& order, uint64_t spill_cap,
-                   Packed& out) {
+                   uint64_t spill_min, Packed& out) {
     // open tiles by free room: FIFO lists (head, tail, next) and a bitset of non-empty rooms
     std::vector<int32_t> head(cap + 1, -1), tail(cap + 1, -1), next;
     std::vector<uint64_t> nonempty(cap / 64 + 1, 0);
@@ -159,7 +164,7 @@ namespace erod {
       }
       uint32_t t;
       if (room < 0) {
-        if (n <= spill_cap) {
+        if (n <= spill_cap && n >= spill_min) {
           out.overflow.push_back(r);
           continue;
         }
@@ -265,18 +270,15 @@ namespace erod {
       set_error("plan_tiles: no tile shape of the requested split lanes and elements per thread");
       return EROD_ERR_INVALID;
     }
-    const uint64_t cap2 = s2 >= 0 ? uint64_t(fused::kShapes[s2].P) * fused::kShapes[s2].NT : 0;
+    // a second group only of tiles smaller than the main ones (a requested
+    // split at least as large as the main tile is ignored); it takes rods of
+    // at most its lanes and at least its elements per thread
+    uint64_t cap2 = s2 >= 0 ? uint64_t(fused::kShapes[s2].P) * fused::kShapes[s2].NT : 0;
+    if (cap2 >= cap) cap2 = 0;
+    const uint64_t min2 = cap2 ? uint64_t(fused::kShapes[s2].P) : 0;
     Packed g1, g2;
-    pack(b, cap, order, cap2, g1);
-    if (!g1.overflow.empty()) {
-      // the second group holds rods of at most cap2 elements, and of at least P2
-      for (const uint32_t r : g1.overflow)
-        if (b->n_active[r] < uint64_t(fused::kShapes[s2].P)) {
-          set_error("plan_tiles: a rod has fewer elements than the split shape's elements per thread");
-          return EROD_ERR_INVALID;
-        }
-      pack(b, cap2, g1.overflow, 0, g2);
-    }
+    pack(b, cap, order, cap2, min2, g1);
+    if (!g1.overflow.empty()) pack(b, cap2, g1.overflow, 0, 0, g2);
     // the kernel stages a tile's rod table in NT entries of LDS (rods have at
     // least two elements, and at least P: a feasible shape never exceeds it)
     for (const Tile& t : g1.tiles)
@@ -458,7 +460,9 @@ namespace erod {
         if ((rc = restore())) break;
       }
     }
-    // the chosen plan is made with the caller's settings and budgets
+    // the chosen plan is made with the caller's settings and budgets; the
+    // chosen main shape is kept for later replans while it stays feasible
+    b->plan.tuned_shape = best;
     b->plan.lanes_per_tile = g.lanes;
     b->plan.elements_per_thread = g.epl;
     b->plan.split_lanes = best_split >= 0 ? fused_shape_lanes(best_split) : g.split_lanes;

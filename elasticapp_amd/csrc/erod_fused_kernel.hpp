@@ -24,7 +24,8 @@
 //
 // Precision
 //   R = double: the reference formulas (acos-based log map with its 1e-10 /
-//   1e-14 offsets, library sin / cos in the director update), its divisions
+//   1e-14 offsets, sin / cos in the director update; acos, sin and cos are
+//   the library's own, erod_math.hpp), its divisions
 //   and its order of evaluation; built without FMA contraction, it rounds
 //   like the op-by-op port (and equals it bitwise on the test batches).
 //   R = float (FP32-bulk): FP64 state, FP64 accumulation of x, v, w and an
@@ -45,6 +46,7 @@
 #pragma once
 
 #include "erod_common.hpp"
+#include "erod_math.hpp"
 
 namespace erod {
   namespace fused {
@@ -107,7 +109,7 @@ namespace erod {
       if (t > start + interval) return 1.0;
       if (kind == EROD_RAMP_LINEAR) return (t - start) / interval;
       const double s = 2.0 * ((t - start) / interval) - 1.0;
-      return 0.5 * (s + (1.0 + (sin(M_PI * s) / M_PI)));
+      return 0.5 * (s + (1.0 + (math::sin(M_PI * s) / M_PI)));
     }
 
     // ---- director update Q <- exp(-[u]x) Q, u = c w, in FP64 ----------------------
@@ -120,8 +122,8 @@ namespace erod {
     __device__ __forceinline__ void rodrigues<double>(double q[9], const double w[3], double cdt) {
       double u0 = cdt * w[0], u1 = cdt * w[1], u2 = cdt * w[2];
       const double theta = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
-      const double beta = sin(theta);
-      const double c_alpha = cos(theta);
+      double beta, c_alpha;
+      math::sincos(theta, &beta, &c_alpha);
       const double alpha = 1.0 - c_alpha;
       const double den = theta + 1e-14;
       u0 /= den;
@@ -187,8 +189,8 @@ namespace erod {
     __device__ __forceinline__ void curvature<double>(const double qc[9], const double qn[9], double D0, double kap[3]) {
       const double tr = qc[0] * qn[0] + qc[1] * qn[1] + qc[2] * qn[2] + qc[3] * qn[3] + qc[4] * qn[4] +
                         qc[5] * qn[5] + qc[6] * qn[6] + qc[7] * qn[7] + qc[8] * qn[8];
-      const double theta = acos(0.5 * tr - (0.5 + 1e-10));
-      const double st = 0.5 * theta / sin(theta + 1e-14) / D0;
+      const double theta = math::acos(0.5 * tr - (0.5 + 1e-10));
+      const double st = 0.5 * theta / math::sin(theta + 1e-14) / D0;
       kap[0] = st * (-qc[3] * qn[6] - qc[4] * qn[7] - qc[5] * qn[8] + qc[6] * qn[3] + qc[7] * qn[4] + qc[8] * qn[5]);
       kap[1] = st * (qc[0] * qn[6] + qc[1] * qn[7] + qc[2] * qn[8] - qc[6] * qn[0] - qc[7] * qn[1] - qc[8] * qn[2]);
       kap[2] = st * (-qc[0] * qn[3] - qc[1] * qn[4] - qc[2] * qn[5] + qc[3] * qn[0] + qc[4] * qn[1] + qc[5] * qn[2]);

@@ -2,7 +2,9 @@
 // timestep, each a grid-stride loop over every slot of the ghost-padded block
 // (ghosts included), in the reference's evaluation order.  This file is built
 // with -ffp-contract=off so that every product and sum is rounded on its own,
-// as in the CPU restatement it is checked against.
+// as in the CPU restatement it is checked against.  acos, sin and cos are the
+// library's own (erod_math.hpp), so that these kernels and a CPU build of the
+// restatement with the same functions compute the same bits.
 //
 // Reference operators (paths under elastica/Systems/CosseratRods/):
 //   spanwise geometry   Components/Geometry/detail/CosseratRodSpanwiseGeometry.hpp:578-664
@@ -14,6 +16,7 @@
 //   operators           Traits/Operations/BlazeBackend/OpsTraits.hpp:88-346
 //   director update     elastica/Systems/States/Expressions/backends/blaze/SO3PrimitiveAddAssign/Scalar.hpp:25-96
 #include "erod_common.hpp"
+#include "erod_math.hpp"
 
 namespace erod {
   namespace {
@@ -51,9 +54,9 @@ namespace erod {
         for (int m = 0; m < 9; ++m) q[m] = Q[m * E + i];
         for (int d = 0; d < 3; ++d) u[d] = c * w[d * E + i];
         const double theta = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-        const double c_alpha = cos(theta);
+        const double c_alpha = math::cos(theta);
         const double alpha = 1.0 - c_alpha;
-        const double beta = sin(theta);
+        const double beta = math::sin(theta);
         for (int d = 0; d < 3; ++d) u[d] /= (theta + 1e-14);
         for (int j = 0; j < 3; ++j) {
           const double com = q[j] * u[0] + q[3 + j] * u[1] + q[6 + j] * u[2];
@@ -217,8 +220,8 @@ namespace erod {
         const double tr = qc[0] * qn[0] + qc[1] * qn[1] + qc[2] * qn[2] + qc[3] * qn[3] + qc[4] * qn[4] +
                           qc[5] * qn[5] + qc[6] * qn[6] + qc[7] * qn[7] + qc[8] * qn[8];
         const double rest = 0.5 + 1e-10;
-        const double theta = acos(0.5 * tr - rest);
-        const double st = 0.5 * theta / sin(theta + 1e-14) / D0[i];
+        const double theta = math::acos(0.5 * tr - rest);
+        const double st = 0.5 * theta / math::sin(theta + 1e-14) / D0[i];
         kap[i] = st * (-qc[3] * qn[6] - qc[4] * qn[7] - qc[5] * qn[8] + qc[6] * qn[3] + qc[7] * qn[4] + qc[8] * qn[5]);
         kap[V + i] = st * (qc[0] * qn[6] + qc[1] * qn[7] + qc[2] * qn[8] - qc[6] * qn[0] - qc[7] * qn[1] - qc[8] * qn[2]);
         kap[2 * V + i] = st * (-qc[0] * qn[3] - qc[1] * qn[4] - qc[2] * qn[5] + qc[3] * qn[0] + qc[4] * qn[1] + qc[5] * qn[2]);
@@ -326,7 +329,7 @@ namespace erod {
       if (t > start + interval) return 1.0;
       if (kind == EROD_RAMP_LINEAR) return (t - start) / interval;
       const double s = 2.0 * ((t - start) / interval) - 1.0;
-      return 0.5 * (s + (1.0 + (sin(M_PI * s) / M_PI)));
+      return 0.5 * (s + (1.0 + (math::sin(M_PI * s) / M_PI)));
     }
 
     __global__ void k_end_force(double* __restrict__ fe, const RodInfo* __restrict__ rods, uint64_t n_rods, uint64_t E, double t) {

@@ -175,8 +175,9 @@ int erod_autotune_fused(erod_batch* b, int precision, int trial_steps, double* b
  * precision.  Keeps x, v, Q, w (and the rod clocks) current; derived fields
  * are refreshed on request by erod_compute_internal_dynamics. */
 int erod_step_fused(erod_batch* b, int precision, uint64_t n);
-/* Same, replaying a captured hipGraph of `chunk` steps n / chunk times
- * (n must be a multiple of chunk). */
+/* Same, replaying a captured hipGraph of `chunk` steps n / chunk times, then
+ * launching the n % chunk steps left directly (in launches of
+ * steps_per_launch, the last one shorter). */
 int erod_step_fused_graph(erod_batch* b, int precision, uint64_t n, uint64_t chunk);
 /* Re-orthonormalise every director (one Newton-Schulz polar step, FP64). */
 int erod_reorthonormalize(erod_batch* b);

@@ -41,12 +41,61 @@
 #include <algorithm>
 #include <cmath>
 #include <cstddef>
+#include <cstdint>
 #include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
+#if defined(ORACLE_DEVICE_MATH)
+#include "../elasticapp_amd/csrc/erod_math.hpp"
+#endif
+
 namespace oracle {
+
+  // The transcendental functions of the FP64 step (acos and sin in
+  // InvRotateDivide, sin and cos in SO3AddAssign, sin in the sin ramp).  The
+  // default build calls the C library's, as the reference does (std::acos,
+  // std::sin, std::cos).  Two other builds of this file exist for the tests
+  // (oracle/Makefile):
+  //   ORACLE_DEVICE_MATH: the device library's own functions
+  //     (elasticapp_amd/csrc/erod_math.hpp, themselves pinned to the C
+  //     library within 1 ulp), so that an FP64 device run can be compared
+  //     with the restatement bit for bit: the only operations whose results
+  //     differ between a C library and the device are these functions;
+  //   ORACLE_PERTURBED_MATH: the C library's results, a fixed quarter of
+  //     them (chosen by a hash of the argument) moved one ulp up or down: a
+  //     stand-in for another C library, to measure how far a trajectory
+  //     carries last-bit differences of these functions.
+  namespace tmath {
+#if defined(ORACLE_DEVICE_MATH)
+    inline double acos(double x) { return erod::math::acos(x); }
+    inline double sin(double x) { return erod::math::sin(x); }
+    inline double cos(double x) { return erod::math::cos(x); }
+#elif defined(ORACLE_PERTURBED_MATH)
+    inline double nudge(double x, double y) {
+      uint64_t h;
+      std::memcpy(&h, &x, sizeof(h));
+      h ^= h >> 33;
+      h *= 0xff51afd7ed558ccdULL;
+      h ^= h >> 33;
+      h *= 0xc4ceb9fe1a85ec53ULL;
+      h ^= h >> 33;
+      switch (h & 7) {
+        case 0: return std::nextafter(y, -INFINITY);
+        case 1: return std::nextafter(y, INFINITY);
+        default: return y;
+      }
+    }
+    inline double acos(double x) { return nudge(x, std::acos(x)); }
+    inline double sin(double x) { return nudge(x, std::sin(x)); }
+    inline double cos(double x) { return nudge(-x, std::cos(x)); }
+#else
+    inline double acos(double x) { return std::acos(x); }
+    inline double sin(double x) { return std::sin(x); }
+    inline double cos(double x) { return std::cos(x); }
+#endif
+  }  // namespace tmath
 
   enum class Stagger { Node, Element, Voronoi };
 
@@ -256,8 +305,8 @@ namespace oracle {
       const T tr = qc[0] * qn[0] + qc[1] * qn[1] + qc[2] * qn[2] + qc[3] * qn[3] + qc[4] * qn[4] +
                    qc[5] * qn[5] + qc[6] * qn[6] + qc[7] * qn[7] + qc[8] * qn[8];
       constexpr T rest(T(0.5) + T(1e-10));
-      const T theta = std::acos(T(0.5) * tr - rest);
-      const T st = T(0.5) * theta / std::sin(theta + T(1e-14)) / D0[i];
+      const T theta = tmath::acos(T(0.5) * tr - rest);
+      const T st = T(0.5) * theta / tmath::sin(theta + T(1e-14)) / D0[i];
       kappa[0 * nv + i] = st * (-qc[3] * qn[6] - qc[4] * qn[7] - qc[5] * qn[8] + qc[6] * qn[3] +
                                 qc[7] * qn[4] + qc[8] * qn[5]);
       kappa[1 * nv + i] = st * (qc[0] * qn[6] + qc[1] * qn[7] + qc[2] * qn[8] - qc[6] * qn[0] -
@@ -274,9 +323,9 @@ namespace oracle {
     T q[9], u[3] = {uin[0], uin[1], uin[2]};
     for (int m = 0; m < 9; ++m) q[m] = Q[m * ne + i];
     const T theta = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-    const T c_alpha = std::cos(theta);
+    const T c_alpha = tmath::cos(theta);
     const T alpha = T(1.0) - c_alpha;
-    const T beta = std::sin(theta);
+    const T beta = tmath::sin(theta);
     for (int d = 0; d < 3; ++d) u[d] /= (theta + T(1e-14));
     for (int j = 0; j < 3; ++j) {
       const T com = q[j] * u[0] + q[3 + j] * u[1] + q[6 + j] * u[2];
@@ -566,7 +615,7 @@ namespace oracle {
     if (t > start + interval) return 1.0;
     if (kind == 1) return (t - start) / interval;
     const double s = 2.0 * ((t - start) / interval) - 1.0;
-    return 0.5 * (s + (1.0 + (std::sin(M_PI * s) / M_PI)));
+    return 0.5 * (s + (1.0 + (tmath::sin(M_PI * s) / M_PI)));
   }
 
   template <typename T>

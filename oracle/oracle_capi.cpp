@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <thread>
 
 #include "cosserat_oracle.hpp"
@@ -29,28 +30,73 @@ namespace {
     ~FpWatch() { g_fp_flags |= std::fetestexcept(FE_INVALID | FE_DIVBYZERO | FE_OVERFLOW); }
   };
 
-  struct Pool {
-    // fan out one task per block, join before returning (the TBB barrier)
+  // Worker threads that live for a whole stepping call, one task per block
+  // per substep (tbb::parallel_for_each keeps its workers the same way); the
+  // return of run() is the barrier between substeps.  Workers wait for the
+  // next substep by spinning, then yielding, so a substep costs no thread
+  // creation.
+  class Pool {
+   public:
+    explicit Pool(int nthreads) : nt_(nthreads > 1 ? nthreads : 1) {
+      for (int t = 1; t < nt_; ++t) th_.emplace_back([this] { work(); });
+    }
+    ~Pool() {
+      stop_.store(true, std::memory_order_release);
+      gen_.fetch_add(1, std::memory_order_acq_rel);
+      for (auto& t : th_) t.join();
+    }
+    Pool(const Pool&) = delete;
+    Pool& operator=(const Pool&) = delete;
+    int threads() const { return nt_; }
+
+    // f(i) for i in [0, n), spread over the workers and the calling thread
     template <typename F>
-    static void for_each(std::vector<oracle::Block<T>>& blocks, int nthreads, F&& f) {
-      if (nthreads <= 1 || blocks.size() <= 1) {
-        for (auto& b : blocks) f(b);
+    void run(std::size_t n, F&& f) {
+      if (nt_ == 1 || n <= 1) {
+        for (std::size_t i = 0; i < n; ++i) f(i);
         return;
       }
-      std::atomic<std::size_t> next{0};
-      auto worker = [&]() {
-        for (;;) {
-          const std::size_t i = next.fetch_add(1);
-          if (i >= blocks.size()) break;
-          f(blocks[i]);
-        }
-      };
-      std::vector<std::thread> th;
-      const int nt = std::min<int>(nthreads, int(blocks.size()));
-      for (int t = 0; t < nt - 1; ++t) th.emplace_back(worker);
-      worker();
-      for (auto& t : th) t.join();
+      std::function<void(std::size_t)> job(std::ref(f));
+      job_ = &job;
+      n_ = n;
+      next_.store(0, std::memory_order_relaxed);
+      busy_.store(nt_ - 1, std::memory_order_relaxed);
+      gen_.fetch_add(1, std::memory_order_acq_rel);  // release the workers
+      drain();
+      while (busy_.load(std::memory_order_acquire) != 0) std::this_thread::yield();
+      job_ = nullptr;
     }
+
+   private:
+    void drain() {
+      for (;;) {
+        const std::size_t i = next_.fetch_add(1, std::memory_order_relaxed);
+        if (i >= n_) break;
+        (*job_)(i);
+      }
+    }
+    void work() {
+      uint64_t seen = 0;
+      for (;;) {
+        uint64_t g;
+        int spins = 0;
+        while ((g = gen_.load(std::memory_order_acquire)) == seen) {
+          if (++spins > 2000) std::this_thread::yield();
+        }
+        seen = g;
+        if (stop_.load(std::memory_order_acquire)) return;
+        drain();
+        busy_.fetch_sub(1, std::memory_order_acq_rel);
+      }
+    }
+    int nt_;
+    std::vector<std::thread> th_;
+    std::atomic<uint64_t> gen_{0};
+    std::atomic<bool> stop_{false};
+    std::atomic<std::size_t> next_{0};
+    std::atomic<int> busy_{0};
+    std::function<void(std::size_t)>* job_ = nullptr;
+    std::size_t n_ = 0;
   };
 }  // namespace
 
@@ -169,6 +215,7 @@ int oracle_kinematic_update(void* h, double dt) {
 
 int oracle_add_clamp(void* h, uint64_t rod, uint64_t index, const double* x3, const double* q9) {
   auto* s = static_cast<Sys*>(h);
+  if (rod >= s->where.size()) return -1;
   oracle::Clamp c{};
   c.rod = rod;
   c.index = index;
@@ -180,6 +227,7 @@ int oracle_add_clamp(void* h, uint64_t rod, uint64_t index, const double* x3, co
 
 int oracle_add_end_force(void* h, uint64_t rod, const double* F3, int ramp, double start, double interval) {
   auto* s = static_cast<Sys*>(h);
+  if (rod >= s->where.size()) return -1;
   oracle::EndForce f{};
   f.rod = rod;
   std::memcpy(f.F, F3, sizeof(f.F));
@@ -231,7 +279,8 @@ void oracle_set_time(void* h, double t) { static_cast<Sys*>(h)->time = t; }
 // n position-Verlet steps with a uniform dt (SymplecticSteppers.hpp:133-217).
 int oracle_step(void* h, double dt, uint64_t n, int nthreads) {
   auto* s = static_cast<Sys*>(h);
-  auto fe = [&](auto&& fn) { Pool::for_each(s->blocks, nthreads, fn); };
+  Pool pool(std::min<int>(nthreads, int(s->blocks.size())));
+  auto fe = [&](auto&& fn) { pool.run(s->blocks.size(), [&](std::size_t i) { fn(s->blocks[i]); }); };
   FpWatch w;
   for (uint64_t i = 0; i < n; ++i) oracle::step(*s, dt, fe);
   return 0;
@@ -247,26 +296,9 @@ int oracle_step_per_block_dt(void* h, const double* block_dt, uint64_t n, int nt
     for (auto const& f : s->forces)
       if (f.ramp != 0) return -1;
   }
+  Pool pool(std::min<int>(nthreads, int(s->blocks.size())));
+  auto run = [&](auto&& fn) { pool.run(s->blocks.size(), [&](std::size_t i) { fn(s->blocks[i], block_dt[i]); }); };
   for (uint64_t it = 0; it < n; ++it) {
-    auto run = [&](auto&& fn) {
-      // fan out over blocks with per-block dt
-      if (nthreads <= 1) {
-        for (std::size_t i = 0; i < s->blocks.size(); ++i) fn(s->blocks[i], block_dt[i]);
-        return;
-      }
-      std::atomic<std::size_t> next{0};
-      auto worker = [&]() {
-        for (;;) {
-          const std::size_t i = next.fetch_add(1);
-          if (i >= s->blocks.size()) break;
-          fn(s->blocks[i], block_dt[i]);
-        }
-      };
-      std::vector<std::thread> th;
-      for (int t = 0; t < nthreads - 1; ++t) th.emplace_back(worker);
-      worker();
-      for (auto& t : th) t.join();
-    };
     run([](oracle::Block<T>& b, double dt) { oracle::kinematic_update(b, dt); });
     oracle::apply_post_kinematic(*s);
     run([](oracle::Block<T>& b, double) { oracle::compute_internal_dynamics(b); });

@@ -26,58 +26,76 @@ VORONOI_FIELDS = ("ReferenceVoronoiLength", "VoronoiLength", "VoronoiDilatation"
                   "InternalCouple", "_DummyVoronoiVector")
 
 
+# builds of the restatement (oracle/Makefile): the C library's sin / cos /
+# acos (the reference's), the device library's, and the C library's with a
+# quarter of the results moved by one ulp
+VARIANTS = {"libm": "liboracle.so", "devmath": "liboracle_devmath.so", "perturbed": "liboracle_perturbed.so"}
+_LOADED = {}
+
+
 def use_library(path=None):
     """Load the oracle from `path` (default: oracle/liboracle.so) for every
-    System created afterwards; the mutation tests load deliberately broken
-    builds this way."""
+    System created afterwards without a `variant`; the mutation tests load
+    deliberately broken builds this way."""
     global _LIB
     _LIB = None
     return _lib(path)
 
 
+def variant_library(variant):
+    """The build of the restatement named in VARIANTS."""
+    path = os.path.join(_HERE, VARIANTS[variant])
+    if path not in _LOADED:
+        _LOADED[path] = _load(path)
+    return _LOADED[path]
+
+
 def _lib(path=None):
     global _LIB
     if _LIB is None:
-        path = path or os.path.join(_HERE, "liboracle.so")
-        if not os.path.exists(path):
-            raise RuntimeError("oracle/liboracle.so is not built: run `make -C oracle`")
-        L = ctypes.CDLL(path)
-        u64, vp, dp, cp = ctypes.c_uint64, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_char_p
-        L.oracle_system_create.restype = vp
-        L.oracle_system_create.argtypes = [ctypes.POINTER(u64), u64, u64, ctypes.c_int]
-        L.oracle_system_destroy.argtypes = [vp]
-        L.oracle_system_n_blocks.restype = u64
-        L.oracle_system_n_blocks.argtypes = [vp]
-        L.oracle_rod_location.argtypes = [vp, u64, ctypes.POINTER(u64), ctypes.POINTER(u64)]
-        L.oracle_block_field.restype = dp
-        L.oracle_block_field.argtypes = [vp, u64, cp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
-        L.oracle_block_fill_ghosts.argtypes = [vp, u64]
-        L.oracle_initialize_straight_rod.argtypes = [vp, u64, dp]
-        for name in ("oracle_compute_internal_forces", "oracle_compute_internal_torques",
-                     "oracle_update_acceleration"):
-            getattr(L, name).argtypes = [vp]
-        L.oracle_kinematic_update.argtypes = [vp, ctypes.c_double]
-        L.oracle_add_clamp.argtypes = [vp, u64, u64, dp, dp]
-        L.oracle_add_end_force.argtypes = [vp, u64, dp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
-        L.oracle_set_gravity.argtypes = [vp, dp]
-        L.oracle_set_plane_contact.argtypes = [vp] + [ctypes.c_double] * 6
-        L.oracle_set_plane_normal.argtypes = [vp, dp]
-        for name in ("oracle_fp_flags", "oracle_fe_invalid", "oracle_fe_divbyzero", "oracle_fe_overflow"):
-            getattr(L, name).restype = ctypes.c_int
-        L.oracle_fp_flags.argtypes = [ctypes.c_int]
-        L.oracle_ramp_value.restype = ctypes.c_double
-        L.oracle_ramp_value.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]
-        L.oracle_time.restype = ctypes.c_double
-        L.oracle_time.argtypes = [vp]
-        L.oracle_set_time.argtypes = [vp, ctypes.c_double]
-        L.oracle_step.argtypes = [vp, ctypes.c_double, u64, ctypes.c_int]
-        L.oracle_step_per_block_dt.argtypes = [vp, dp, u64, ctypes.c_int]
-        L.oracle_two_point_difference.argtypes = [dp, u64, dp]
-        L.oracle_quadrature.argtypes = [dp, u64, dp]
-        L.oracle_inv_rotate_divide.argtypes = [dp, u64, dp, dp]
-        L.oracle_so3_add_assign.argtypes = [dp, u64, dp]
-        _LIB = L
+        _LIB = _load(path or os.path.join(_HERE, "liboracle.so"))
     return _LIB
+
+
+def _load(path):
+    if not os.path.exists(path):
+        raise RuntimeError("%s is not built: run `make -C oracle`" % os.path.relpath(path, _HERE))
+    L = ctypes.CDLL(path)
+    u64, vp, dp, cp = ctypes.c_uint64, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_char_p
+    L.oracle_system_create.restype = vp
+    L.oracle_system_create.argtypes = [ctypes.POINTER(u64), u64, u64, ctypes.c_int]
+    L.oracle_system_destroy.argtypes = [vp]
+    L.oracle_system_n_blocks.restype = u64
+    L.oracle_system_n_blocks.argtypes = [vp]
+    L.oracle_rod_location.argtypes = [vp, u64, ctypes.POINTER(u64), ctypes.POINTER(u64)]
+    L.oracle_block_field.restype = dp
+    L.oracle_block_field.argtypes = [vp, u64, cp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
+    L.oracle_block_fill_ghosts.argtypes = [vp, u64]
+    L.oracle_initialize_straight_rod.argtypes = [vp, u64, dp]
+    for name in ("oracle_compute_internal_forces", "oracle_compute_internal_torques",
+                 "oracle_update_acceleration"):
+        getattr(L, name).argtypes = [vp]
+    L.oracle_kinematic_update.argtypes = [vp, ctypes.c_double]
+    L.oracle_add_clamp.argtypes = [vp, u64, u64, dp, dp]
+    L.oracle_add_end_force.argtypes = [vp, u64, dp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+    L.oracle_set_gravity.argtypes = [vp, dp]
+    L.oracle_set_plane_contact.argtypes = [vp] + [ctypes.c_double] * 6
+    L.oracle_set_plane_normal.argtypes = [vp, dp]
+    for name in ("oracle_fp_flags", "oracle_fe_invalid", "oracle_fe_divbyzero", "oracle_fe_overflow"):
+        getattr(L, name).restype = ctypes.c_int
+    L.oracle_fp_flags.argtypes = [ctypes.c_int]
+    L.oracle_ramp_value.restype = ctypes.c_double
+    L.oracle_ramp_value.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    L.oracle_time.restype = ctypes.c_double
+    L.oracle_time.argtypes = [vp]
+    L.oracle_set_time.argtypes = [vp, ctypes.c_double]
+    L.oracle_step.argtypes = [vp, ctypes.c_double, u64, ctypes.c_int]
+    L.oracle_step_per_block_dt.argtypes = [vp, dp, u64, ctypes.c_int]
+    L.oracle_two_point_difference.argtypes = [dp, u64, dp]
+    L.oracle_quadrature.argtypes = [dp, u64, dp]
+    L.oracle_inv_rotate_divide.argtypes = [dp, u64, dp, dp]
+    L.oracle_so3_add_assign.argtypes = [dp, u64, dp]
+    return L
 
 
 def _dptr(a):
@@ -101,8 +119,9 @@ class System:
     """Rods packed first-fit into ghost-padded blocks of <= max_block_elems
     elements (0: one block per rod)."""
 
-    def __init__(self, n_elems, max_block_elems=1024, damped=True):
-        L = self._L = _lib()  # a System keeps the library it was created with
+    def __init__(self, n_elems, max_block_elems=1024, damped=True, variant=None):
+        # a System keeps the library it was created with
+        L = self._L = _lib() if variant is None else variant_library(variant)
         self.n_elems = [int(n) for n in n_elems]
         arr = (ctypes.c_uint64 * len(self.n_elems))(*self.n_elems)
         self._h = L.oracle_system_create(arr, len(self.n_elems), int(max_block_elems), 1 if damped else 0)
@@ -167,11 +186,13 @@ class System:
     def add_clamp(self, rod, x, Q, index=0):
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(3)
         Q = np.ascontiguousarray(Q, dtype=np.float64).reshape(9)
-        self._L.oracle_add_clamp(self._h, rod, index, _dptr(x), _dptr(Q))
+        if self._L.oracle_add_clamp(self._h, rod, index, _dptr(x), _dptr(Q)) != 0:
+            raise IndexError("clamp on rod %d of a system of %d rods" % (rod, len(self.n_elems)))
 
     def add_end_force(self, rod, F, ramp=0, start=0.0, interval=1.0):
         F = np.ascontiguousarray(F, dtype=np.float64).reshape(3)
-        self._L.oracle_add_end_force(self._h, rod, _dptr(F), int(ramp), float(start), float(interval))
+        if self._L.oracle_add_end_force(self._h, rod, _dptr(F), int(ramp), float(start), float(interval)) != 0:
+            raise IndexError("end force on rod %d of a system of %d rods" % (rod, len(self.n_elems)))
 
     def set_gravity(self, g):
         g = np.ascontiguousarray(g, dtype=np.float64).reshape(3)

@@ -72,8 +72,10 @@ def device_block(wl, damped=True):
     return build_block(wl, damped=damped)
 
 
-def oracle_system(wl, damped=True, max_block_elems=0):
-    s = pyoracle.System(wl.n_elems, max_block_elems=max_block_elems, damped=damped)
+def oracle_system(wl, damped=True, max_block_elems=0, variant=None):
+    """The workload in an oracle System; `variant` names a build of the
+    restatement (pyoracle.VARIANTS; default: the C library's functions)."""
+    s = pyoracle.System(wl.n_elems, max_block_elems=max_block_elems, damped=damped, variant=variant)
     for r, st in enumerate(wl.states):
         for tag, a in st.items():
             s.rod_field(r, tag)[...] = a

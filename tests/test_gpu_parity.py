@@ -192,8 +192,8 @@ def test_edge_lengths_are_independent_of_placement(precision):
 def test_rejected_arguments_leave_the_batch_as_it_was():
     """Bad arguments raise and change nothing: a ramped force with an empty
     interval (its ramp would be 0/0 at the start time), a rod dt that is not
-    positive (the call is all or nothing), a graph length that is not a whole
-    number of chunks.  The batch then steps bitwise like an untouched one."""
+    positive (the call is all or nothing), a graph chunk of zero steps.  The
+    batch then steps bitwise like an untouched one."""
     wl = random_workload(4, 8, 40, seed=71, steps=50)
     a, b = device_block(wl), device_block(wl)
     dts = np.asarray([wl.dt * (1.0 - 0.1 * r) for r in range(4)])
@@ -207,8 +207,8 @@ def test_rejected_arguments_leave_the_batch_as_it_was():
     bad[2] = -1.0
     with pytest.raises(ErodError, match="positive"):
         a.set_rod_dt(bad)
-    with pytest.raises(ErodError, match="multiple"):
-        a.step_fused_graph(30, 20, FP64)
+    with pytest.raises(ErodError, match="chunk"):
+        a.step_fused_graph(30, 0, FP64)
     for blk in (a, b):
         blk.step_fused(wl.steps, FP64)
     for r in range(4):
@@ -274,6 +274,28 @@ def test_fused_graph_replay_equals_direct():
     for r in range(len(wl.n_elems)):
         for t in STATE:
             assert np.array_equal(a.get_rod(t, r), b.get_rod(t, r)), (r, t)
+
+
+@pytest.mark.parametrize("precision", [FP32_BULK, FP64])
+def test_fused_graph_takes_steps_that_do_not_fill_a_chunk(precision):
+    """n not a multiple of the chunk: the whole chunks are replayed and the
+    rest launched directly with the chunk's launch lengths, so the run equals
+    the direct path over the same steps (also with fewer steps than one
+    chunk), and the launch count covers both parts."""
+    wl = random_workload(20, 2, 120, seed=17, steps=0)
+    for n, chunk in ((75, 20), (5, 20), (20, 20)):
+        a, b = device_block(wl), device_block(wl)
+        for blk in (a, b):
+            blk.set_fused_config(0, 0, 10)
+        a.step_fused(n, precision)
+        b.step_fused_graph(n, chunk, precision)
+        assert b.last_kernel_ms()[1] == -(-n // 10), (n, chunk)
+        for r in range(len(wl.n_elems)):
+            for t in STATE:
+                assert np.array_equal(a.get_rod(t, r), b.get_rod(t, r)), (n, chunk, r, t)
+        assert np.array_equal(a.rod_time(), b.rod_time()), (n, chunk)
+        a.close()
+        b.close()
 
 
 @pytest.mark.parametrize("precision", [FP32_BULK, FP64])
