@@ -76,6 +76,8 @@ def load_library():
         "erod_download_rod": (i32, [vp, i32, u64, P(dbl)]),
         "erod_fill_ghosts": (i32, [vp]),
         "erod_kinematic_update": (i32, [vp, dbl]),
+        "erod_kinematic_update_by": (i32, [vp, dbl]),
+        "erod_fill_field": (i32, [vp, i32, dbl]),
         "erod_compute_internal_dynamics": (i32, [vp, dbl]),
         "erod_dynamic_update": (i32, [vp, dbl]),
         "erod_add_clamp": (i32, [vp, u64, u64, P(dbl), P(dbl)]),
@@ -243,6 +245,14 @@ class DeviceBlock:
     # -- the three per-block stepper calls (op-by-op FP64) --
     def kinematic_update(self, dt):
         _check(_LIB.erod_kinematic_update(self._h, float(dt)))
+
+    def kinematic_update_by(self, c):
+        """x += c v, Q <- exp(-c [w]x) Q (kinematic_states() += c * kinematic_rates())."""
+        _check(_LIB.erod_kinematic_update_by(self._h, float(c)))
+
+    def fill_field(self, tag, value):
+        """Every slot of a field, ghosts included, set to `value`."""
+        _check(_LIB.erod_fill_field(self._h, FIELDS[tag][0], float(value)))
 
     def compute_internal_dynamics(self, time=0.0):
         _check(_LIB.erod_compute_internal_dynamics(self._h, float(time)))

@@ -336,6 +336,18 @@ int erod_kinematic_update(erod_batch* b, double dt) {
   return ops_kinematic_update(b, dt);
 }
 
+int erod_kinematic_update_by(erod_batch* b, double c) {
+  if (int rc = check_plain(b)) return rc;
+  return ops_kinematic_update_by(b, c);
+}
+
+int erod_fill_field(erod_batch* b, int field, double value) {
+  if (check_batch(b) || check_field(field)) return EROD_ERR_INVALID;
+  int rc = ops_fill_field(b, field, value);
+  if (rc == EROD_OK) mark_field_written(b, field);
+  return rc;
+}
+
 int erod_compute_internal_dynamics(erod_batch* b, double time) {
   (void)time;
   if (int rc = check_plain(b)) return rc;

@@ -188,6 +188,8 @@ namespace erod {
 
   // op-by-op path (erod_ops.hip)
   int ops_kinematic_update(erod_batch* b, double dt);
+  int ops_kinematic_update_by(erod_batch* b, double c);
+  int ops_fill_field(erod_batch* b, int field, double v);
   int ops_compute_internal_dynamics(erod_batch* b);
   int ops_dynamic_update(erod_batch* b, double dt);
   int ops_fill_ghosts(erod_batch* b, int field);

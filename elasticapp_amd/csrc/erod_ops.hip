@@ -405,12 +405,23 @@ namespace erod {
     return hip_check(hipGetLastError(), "fill_ghosts");
   }
 
-  int ops_kinematic_update(erod_batch* b, double dt) {
+  // kinematic_states() += c * kinematic_rates(): x += c v, Q <- exp(-c [w]x) Q
+  int ops_kinematic_update_by(erod_batch* b, double c) {
     const uint64_t E = b->E, N = E + 1;
-    const double c = 0.5 * dt;
     L(k_axpy, 3 * N, F(EROD_POSITION), F(EROD_VELOCITY), c, 3 * N);
     L(k_so3, E, F(EROD_DIRECTOR), F(EROD_ANGULAR_VELOCITY), c, E);
     return hip_check(hipGetLastError(), "kinematic_update");
+  }
+
+  // PositionVerlet's kinematic substep: the coefficient (1/2) dt
+  int ops_kinematic_update(erod_batch* b, double dt) { return ops_kinematic_update_by(b, 0.5 * dt); }
+
+  // every slot of a field (ghosts included) set to one value: the block-wide
+  // assignment of a scalar to a variable
+  int ops_fill_field(erod_batch* b, int field, double v) {
+    const uint64_t n = uint64_t(field_info(field).ncomp) * b->field_slots(field);
+    L(k_fill, n, b->d_field[field], v, n);
+    return hip_check(hipGetLastError(), "fill_field");
   }
 
   int ops_zero_external(erod_batch* b) {

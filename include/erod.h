@@ -101,6 +101,15 @@ int erod_fill_ghosts(erod_batch* b);
 /* PositionVerlet::KinematicSubStep (PositionVerlet.hpp:60-69): x += (dt/2) v,
  * Q <- exp(-(dt/2)[w]x) Q over every slot. */
 int erod_kinematic_update(erod_batch* b, double dt);
+/* The same update with any coefficient: x += c v, Q <- exp(-c [w]x) Q, that
+ * is sys.kinematic_states() += c * sys.kinematic_rates(t) for the
+ * coefficient a symplectic algorithm forms (PositionVerlet.hpp:60-69 uses
+ * c = dt / 2, VelocityVerlet.hpp:74 c = dt). */
+int erod_kinematic_update_by(erod_batch* b, double c);
+/* Every slot of a field (ghosts included) set to `value`: the block-wide
+ * assignment of a scalar to a variable, as in external_loads(system) = 0.0
+ * (SymplecticSteppers.hpp:178-186; Access.hpp:229-247). */
+int erod_fill_field(erod_batch* b, int field, double value);
 /* ExternalLoads = ExternalTorques = 0, then compute_internal_dynamics
  * (SymplecticSteppers.hpp:178-188, ElasticityInterface.hpp:360-365). */
 int erod_compute_internal_dynamics(erod_batch* b, double time);
