@@ -96,9 +96,12 @@ def load_library():
         "erod_get_fused_config": (i32, [vp, P(i32), P(i32), P(i32)]),
         "erod_set_fused_split": (i32, [vp, i32, i32]),
         "erod_get_fused_split": (i32, [vp, P(i32), P(i32), P(u64)]),
+        "erod_set_fused_pieces": (i32, [vp, i32, i32, i32]),
+        "erod_get_fused_pieces": (i32, [vp, P(i32), P(i32), P(i32), P(u64), P(u64)]),
         "erod_autotune_fused": (i32, [vp, i32, i32, P(dbl)]),
         "erod_step_fused": (i32, [vp, i32, u64]),
         "erod_step_fused_graph": (i32, [vp, i32, u64, u64]),
+        "erod_capture_fused_graph": (i32, [vp, i32, u64]),
         "erod_reorthonormalize": (i32, [vp]),
         "erod_last_kernel_ms": (i32, [vp, P(dbl), P(u64)]),
         "erod_synchronize": (i32, [vp]),
@@ -325,6 +328,18 @@ class DeviceBlock:
         _check(_LIB.erod_get_fused_split(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(t)))
         return a.value, b.value, t.value
 
+    def set_fused_pieces(self, lanes_per_tile=0, elements_per_thread=0, halo=0):
+        """Rods longer than the main tile (or than `lanes_per_tile`) run as
+        pieces of that shape recomputing `halo` elements of each neighbour."""
+        _check(_LIB.erod_set_fused_pieces(self._h, int(lanes_per_tile), int(elements_per_thread), int(halo)))
+
+    def fused_pieces(self):
+        """(lanes, elements per thread, halo, tiles, pieces) of the current plan's pieces."""
+        a, b, h, t, p = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_uint64(), ctypes.c_uint64()
+        _check(_LIB.erod_get_fused_pieces(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(h), ctypes.byref(t),
+                                          ctypes.byref(p)))
+        return a.value, b.value, h.value, t.value, p.value
+
     def fused_config(self):
         """(lanes_per_tile, elements_per_thread, steps_per_launch) of the current plan."""
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
@@ -342,6 +357,10 @@ class DeviceBlock:
 
     def step_fused_graph(self, n, chunk, precision=FP64):
         _check(_LIB.erod_step_fused_graph(self._h, int(precision), int(n), int(chunk)))
+
+    def capture_fused_graph(self, chunk, precision=FP64):
+        """Capture the graph of one chunk of fused steps now, without stepping."""
+        _check(_LIB.erod_capture_fused_graph(self._h, int(precision), int(chunk)))
 
     def reorthonormalize(self):
         _check(_LIB.erod_reorthonormalize(self._h))

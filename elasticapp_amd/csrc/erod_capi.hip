@@ -66,7 +66,9 @@ namespace erod {
   }
 
   // per-rod damping rates for the fused path: the reference stores one rate per
-  // node / element; the fused kernels need it uniform along each rod
+  // node / element (ExplicitDampingAdapterFacade.hpp:185-198); a rod whose
+  // rates are the same along it carries them in its record, any other is
+  // flagged (bit 2) and the fused kernels read its rates per node and element
   static int refresh_rod_damping(erod_batch* b) {
     const uint64_t N = b->E + 1, E = b->E;
     std::vector<double> gf(N), gt(E);
@@ -76,14 +78,14 @@ namespace erod {
     for (uint64_t r = 0; r < b->n_rods; ++r) {
       const uint64_t s = b->starts[r], n = b->n_active[r];
       double f = b->damped && n ? gf[s] : 0.0, t = b->damped && n ? gt[s] : 0.0;
+      bool uniform = true;
       if (b->damped && n) {
-        for (uint64_t i = 0; i <= n; ++i)
-          if (gf[s + i] != f) { set_error("fused path needs a uniform ForceDampingRate per rod"); return EROD_ERR_UNSUPPORTED; }
-        for (uint64_t i = 0; i < n; ++i)
-          if (gt[s + i] != t) { set_error("fused path needs a uniform TorqueDampingRate per rod"); return EROD_ERR_UNSUPPORTED; }
+        for (uint64_t i = 0; i <= n && uniform; ++i) uniform = gf[s + i] == f;
+        for (uint64_t i = 0; i < n && uniform; ++i) uniform = gt[s + i] == t;
       }
-      b->rods[r].gamma_f = f;
-      b->rods[r].gamma_t = t;
+      b->rods[r].gamma_f = uniform ? f : 0.0;
+      b->rods[r].gamma_t = uniform ? t : 0.0;
+      b->rods[r].flags = uniform ? (b->rods[r].flags & ~4u) : (b->rods[r].flags | 4u);
     }
     b->rods_dirty = true;
     return sync_rods(b);
@@ -98,7 +100,8 @@ namespace {
     return f == EROD_MASS || f == EROD_INV_MASS || f == EROD_REFERENCE_ELEMENT_LENGTH ||
            f == EROD_REFERENCE_SHEAR_STRETCH_STRAIN || f == EROD_SHEAR_STRETCH_RIGIDITY ||
            f == EROD_MASS_SECOND_MOMENT_OF_INERTIA || f == EROD_INV_MASS_SECOND_MOMENT_OF_INERTIA ||
-           f == EROD_REFERENCE_VORONOI_LENGTH || f == EROD_REFERENCE_CURVATURE || f == EROD_BENDING_TWIST_RIGIDITY;
+           f == EROD_REFERENCE_VORONOI_LENGTH || f == EROD_REFERENCE_CURVATURE || f == EROD_BENDING_TWIST_RIGIDITY ||
+           f == EROD_FORCE_DAMPING_RATE || f == EROD_TORQUE_DAMPING_RATE;
   }
 
   int check_batch(const erod_batch* b) {
@@ -236,12 +239,15 @@ int erod_batch_destroy(erod_batch* b) {
   hipFree(b->d_time);
   hipFree(b->d_flags);
   hipFree(b->d_const32);
-  hipFree(b->plan.d_tiles);
-  hipFree(b->plan.d_tile_rods);
-  hipFree(b->plan.d_tile_lane0);
-  hipFree(b->plan.d_tiles2);
-  hipFree(b->plan.d_tile_rods2);
-  hipFree(b->plan.d_tile_lane02);
+  for (TileGroup& g : b->plan.g) {
+    hipFree(g.d_tiles);
+    hipFree(g.d_tile_rods);
+    hipFree(g.d_tile_lane0);
+    hipFree(g.d_segs);
+    hipFree(g.d_seg_rods);
+  }
+  hipFree(b->d_time_next);
+  for (double* p : b->d_shadow) hipFree(p);
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);
   if (b->stream) hipStreamDestroy(b->stream);
@@ -663,16 +669,42 @@ int erod_set_fused_split(erod_batch* b, int lanes_per_tile, int elements_per_thr
 
 int erod_get_fused_split(const erod_batch* b, int* lanes, int* elements_per_thread, uint64_t* tiles) {
   if (check_batch(b)) return EROD_ERR_INVALID;
-  const int si = b->plan.valid ? b->plan.shape2 : -1;
+  const int si = b->plan.valid ? b->plan.g[1].shape : -1;
   if (lanes) *lanes = si >= 0 ? fused_shape_lanes(si) : 0;
   if (elements_per_thread) *elements_per_thread = si >= 0 ? fused_shape_epl(si) : 0;
-  if (tiles) *tiles = si >= 0 ? b->plan.n_tiles2 : 0;
+  if (tiles) *tiles = si >= 0 ? b->plan.g[1].n_tiles : 0;
+  return EROD_OK;
+}
+
+int erod_set_fused_pieces(erod_batch* b, int lanes_per_tile, int elements_per_thread, int halo) {
+  if (check_batch(b) || lanes_per_tile < 0 || elements_per_thread < 0 || halo < 0 || halo == 1) {
+    set_error("erod_set_fused_pieces: negative argument, or a halo of 1");
+    return EROD_ERR_INVALID;
+  }
+  b->plan.seg_lanes = lanes_per_tile;
+  b->plan.seg_epl = elements_per_thread;
+  b->plan.halo = halo ? halo : 16;
+  b->plan.valid = false;
+  b->graph_precision = -1;
+  return EROD_OK;
+}
+
+int erod_get_fused_pieces(const erod_batch* b, int* lanes, int* elements_per_thread, int* halo, uint64_t* tiles,
+                          uint64_t* pieces) {
+  if (check_batch(b)) return EROD_ERR_INVALID;
+  const TileGroup& g = b->plan.g[2];
+  const int si = b->plan.valid ? g.shape : -1;
+  if (lanes) *lanes = si >= 0 ? fused_shape_lanes(si) : 0;
+  if (elements_per_thread) *elements_per_thread = si >= 0 ? fused_shape_epl(si) : 0;
+  if (halo) *halo = b->plan.halo;
+  if (tiles) *tiles = si >= 0 ? g.n_tiles : 0;
+  if (pieces) *pieces = si >= 0 ? g.n_segs : 0;
   return EROD_OK;
 }
 
 int erod_get_fused_config(const erod_batch* b, int* lanes, int* elements_per_thread, int* steps) {
   if (check_batch(b)) return EROD_ERR_INVALID;
-  const int si = b->plan.valid ? b->plan.shape : -1;
+  const int si = b->plan.valid ? b->plan.g[0].shape : -1;
   if (lanes) *lanes = si >= 0 ? fused_shape_lanes(si) : 0;
   if (elements_per_thread) *elements_per_thread = si >= 0 ? fused_shape_epl(si) : 0;
   if (steps) *steps = b->plan.steps_per_launch > 0 ? b->plan.steps_per_launch : 100;
@@ -705,16 +737,10 @@ int erod_step_fused(erod_batch* b, int precision, uint64_t n) {
   return EROD_OK;
 }
 
-int erod_step_fused_graph(erod_batch* b, int precision, uint64_t n, uint64_t chunk) {
-  if (check_batch(b) || chunk == 0) {
-    set_error("erod_step_fused_graph: chunk must be positive");
-    return EROD_ERR_INVALID;
-  }
-  int rc = fused_common(b, precision);
-  if (rc) return rc;
+// capture (or keep) the graph of one chunk of `chunk` fused steps
+static int ensure_graph(erod_batch* b, int precision, uint64_t chunk) {
   const uint64_t spl = b->plan.steps_per_launch > 0 ? uint64_t(b->plan.steps_per_launch) : 100;
-  const uint64_t whole = n / chunk, rest = n % chunk;
-  if (whole > 0 && (b->graph_precision != precision || b->graph_chunk != chunk || !b->graph_exec)) {
+  if (b->graph_precision != precision || b->graph_chunk != chunk || !b->graph_exec) {
     if (b->graph_exec) {
       hipGraphExecDestroy(b->graph_exec);
       b->graph_exec = nullptr;
@@ -744,6 +770,28 @@ int erod_step_fused_graph(erod_batch* b, int precision, uint64_t n, uint64_t chu
     b->graph_chunk = chunk;
     b->graph_launches = launches;
   }
+  return EROD_OK;
+}
+
+int erod_capture_fused_graph(erod_batch* b, int precision, uint64_t chunk) {
+  if (check_batch(b) || chunk == 0) {
+    set_error("erod_capture_fused_graph: chunk must be positive");
+    return EROD_ERR_INVALID;
+  }
+  int rc = fused_common(b, precision);
+  return rc ? rc : ensure_graph(b, precision, chunk);
+}
+
+int erod_step_fused_graph(erod_batch* b, int precision, uint64_t n, uint64_t chunk) {
+  if (check_batch(b) || chunk == 0) {
+    set_error("erod_step_fused_graph: chunk must be positive");
+    return EROD_ERR_INVALID;
+  }
+  int rc = fused_common(b, precision);
+  if (rc) return rc;
+  const uint64_t spl = b->plan.steps_per_launch > 0 ? uint64_t(b->plan.steps_per_launch) : 100;
+  const uint64_t whole = n / chunk, rest = n % chunk;
+  if (whole > 0 && (rc = ensure_graph(b, precision, chunk))) return rc;
   EROD_HIP(hipEventRecord(b->ev0, b->stream));
   for (uint64_t i = 0; i < whole; ++i) {
     EROD_HIP(hipGraphLaunch(b->graph_exec, b->stream));

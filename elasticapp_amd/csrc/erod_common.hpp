@@ -55,10 +55,10 @@ namespace erod {
   struct RodInfo {
     uint64_t start;   // slot of node 0 / element 0 / voronoi 0
     uint32_t n;       // elements
-    uint32_t flags;   // bit0 clamp at index 0, bit1 end force
+    uint32_t flags;   // bit0 clamp at index 0, bit1 end force, bit2 damping rates that vary along the rod
     double dt;        // timestep of this rod (fused path)
-    double gamma_f;   // force damping rate (uniform per rod)
-    double gamma_t;   // torque damping rate (uniform per rod)
+    double gamma_f;   // force damping rate (when the same along the rod)
+    double gamma_t;   // torque damping rate (when the same along the rod)
     double clamp_x[3];
     double clamp_q[9];
     double force[3];
@@ -82,26 +82,51 @@ namespace erod {
     uint32_t pad;
   };
 
+  // A piece of a rod that one tile holds: elements [off, off + n) of rod
+  // `rod`.  The tile writes back only the elements [off + w0, off + w1) it
+  // owns (and their nodes); the others are a halo, the neighbouring pieces'
+  // elements, recomputed from the state they had when the launch began.  In
+  // one step a node or element depends on one neighbour before it and two
+  // after it, so a halo of h elements on each side keeps the owned elements
+  // exact for h / 2 steps: a launch over pieces takes at most that many.
+  // `ends`: kSegFirst if the piece starts at the rod's first element,
+  // kSegLast if it ends at the rod's last (the rod end rules, clamp and end
+  // force apply only at such ends).
+  struct Seg {
+    uint32_t rod, off, n, w0, w1, ends;
+  };
+  constexpr uint32_t kSegFirst = 1u, kSegLast = 2u;
+
+  // One group of tiles of a fused plan, all of one shape.
+  struct TileGroup {
+    int shape = -1;                    // index into fused::kShapes (-1: no tiles)
+    uint64_t n_tiles = 0;
+    Tile* d_tiles = nullptr;
+    uint32_t* d_tile_rods = nullptr;   // rod ids (or, with pieces, piece ids) in tile order
+    uint32_t* d_tile_lane0 = nullptr;  // first lane of each within its tile
+    uint64_t capacity = 0;             // entries allocated in each of the three arrays
+    // pieces of long rods (nullptr: the entries are whole rods)
+    Seg* d_segs = nullptr;
+    uint64_t n_segs = 0, seg_capacity = 0;
+    uint32_t* d_seg_rods = nullptr;    // the rods cut into the pieces
+    uint64_t n_seg_rods = 0, seg_rod_capacity = 0;
+    int max_steps = 0;                 // steps per launch at most (0: no limit)
+  };
+
   struct FusedPlan {
     int lanes_per_tile = 0;       // requested elements per tile (0 = any)
     int elements_per_thread = 0;  // requested elements per thread (0 = any)
     int steps_per_launch = 0;     // requested (0 = 100)
-    int shape = -1;               // chosen index into fused::kShapes
     int tuned_shape = -1;         // the autotuner's choice (-1: none), preferred while feasible
-    uint64_t n_tiles = 0;
-    uint32_t* d_tile_rods = nullptr;  // rod ids in tile order
-    uint32_t* d_tile_lane0 = nullptr; // first lane of each rod within its tile
-    Tile* d_tiles = nullptr;
-    uint64_t rod_capacity = 0;        // entries allocated in each of the three arrays
     // optional second group: rods of at most split_lanes elements that find no
     // room beside longer rods run in tiles of that shape, in a second launch
     int split_lanes = 0, split_epl = 0;  // requested (0 = no second group)
-    int shape2 = -1;                     // its index into fused::kShapes
-    uint64_t n_tiles2 = 0;
-    uint32_t* d_tile_rods2 = nullptr;
-    uint32_t* d_tile_lane02 = nullptr;
-    Tile* d_tiles2 = nullptr;
-    uint64_t rod_capacity2 = 0;
+    // rods longer than the main tile, or than seg_lanes when it is set, run
+    // as overlapping pieces of at most seg_lanes (0: the main tile's) lanes
+    // with `halo` elements recomputed on each side
+    int seg_lanes = 0, seg_epl = 0, halo = 16;
+    // groups: 0 main, 1 second (short rods), 2 pieces of long rods
+    TileGroup g[3];
     bool valid = false;
   };
 
@@ -148,6 +173,9 @@ struct erod_batch {
   uint64_t graph_launches = 0;  // kernel launches in one replay of graph_exec
   int graph_precision = -1;
   uint64_t graph_chunk = 0;
+  double* d_time_next = nullptr;  // rod clocks written by launches over pieces, copied back after each
+  // second copy of x, v, Q, w that launches over pieces alternate with (same layout as the fields)
+  double* d_shadow[4] = {};
   // timing of the dominant kernel
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   uint64_t last_launches = 0;

@@ -13,7 +13,8 @@ namespace erod {
   static const int kConst32Fields[] = {
       EROD_MASS, EROD_INV_MASS, EROD_REFERENCE_ELEMENT_LENGTH, EROD_REFERENCE_SHEAR_STRETCH_STRAIN,
       EROD_SHEAR_STRETCH_RIGIDITY, EROD_MASS_SECOND_MOMENT_OF_INERTIA, EROD_INV_MASS_SECOND_MOMENT_OF_INERTIA,
-      EROD_REFERENCE_VORONOI_LENGTH, EROD_REFERENCE_CURVATURE, EROD_BENDING_TWIST_RIGIDITY};
+      EROD_REFERENCE_VORONOI_LENGTH, EROD_REFERENCE_CURVATURE, EROD_BENDING_TWIST_RIGIDITY,
+      EROD_FORCE_DAMPING_RATE, EROD_TORQUE_DAMPING_RATE};
   static constexpr int kNConst32 = sizeof(kConst32Fields) / sizeof(kConst32Fields[0]);
 
   // offset (in floats) of constant i; const32_offset(b, kNConst32) is the total
@@ -55,9 +56,10 @@ namespace erod {
       }
     }
 
+    template <typename Flag>
     __global__ void k_check_finite(const double* __restrict__ X, const double* __restrict__ Vel, const double* __restrict__ Qd,
                                    const double* __restrict__ W, const RodInfo* __restrict__ rods, uint64_t n_rods,
-                                   uint64_t E, int* __restrict__ flags) {
+                                   uint64_t E, Flag* __restrict__ flags) {
       const uint64_t N = E + 1;
       for (uint64_t r = blockIdx.x; r < n_rods; r += gridDim.x) {
         const RodInfo ri = rods[r];
@@ -70,7 +72,7 @@ namespace erod {
             for (int m = 0; m < 9; ++m) bad |= !isfinite(Qd[m * E + s]);
           }
         }
-        if (bad) flags[r] = 1;
+        if (bad) flags[r] = Flag(1);
       }
     }
 
@@ -86,63 +88,82 @@ namespace erod {
     return b->n_active[r] != 0 && (!b->budgets || b->steps_left[r] != 0);
   }
 
-  // Feasible shapes: capacity P * NT holds the longest rod, and every rod has
-  // at least P elements (a thread then holds at most one rod end).
+  // the shape of the pieces of long rods: the one of seg_lanes lanes (and
+  // seg_epl elements per thread when set), or the main shape; -2 if none
+  static int piece_shape(const erod_batch* b, int main_shape) {
+    if (b->plan.seg_lanes <= 0) return main_shape;
+    for (int si : {0, 1, 2, 4, 5, 3, 7, 6})  // the default order of preference
+      if (fused_shape_lanes(si) == b->plan.seg_lanes && (b->plan.seg_epl <= 0 || fused_shape_epl(si) == b->plan.seg_epl))
+        return si;
+    return -2;
+  }
+
+  // A rod runs as pieces when it is longer than the main tile, or longer
+  // than the pieces' tile when seg_lanes is set.
+  static bool cut_rod(const erod_batch* b, uint64_t n, int main_shape) {
+    const uint64_t cap = uint64_t(fused_shape_lanes(main_shape));
+    return n > cap || (b->plan.seg_lanes > 0 && n > uint64_t(b->plan.seg_lanes));
+  }
+
+  // Feasible main shapes: its lanes and elements per thread are the ones
+  // requested, and every rod that is not cut has at least P elements (a
+  // thread then holds at most one rod end).
   static bool shape_feasible(const erod_batch* b, int si) {
-    uint64_t maxn = 0, minn = ~uint64_t(0);
-    for (uint64_t r = 0; r < b->n_rods; ++r) {
-      if (!planned(b, r)) continue;
-      const uint64_t n = b->n_active[r];
-      maxn = n > maxn ? n : maxn;
-      minn = n < minn ? n : minn;
-    }
-    if (maxn == 0) minn = maxn;  // nothing to run: every shape is feasible
     const fused::Shape sh = fused::kShapes[si];
     const int lanes = b->plan.lanes_per_tile, epl = b->plan.elements_per_thread;
     if (lanes && sh.P * sh.NT != lanes) return false;
     if (epl && sh.P != epl) return false;
-    return uint64_t(sh.P) * sh.NT >= maxn && minn >= uint64_t(sh.P);
+    for (uint64_t r = 0; r < b->n_rods; ++r)
+      if (planned(b, r) && !cut_rod(b, b->n_active[r], si) && b->n_active[r] < uint64_t(sh.P)) return false;
+    return true;
   }
 
   // Default shape without measurement: the smallest capacity that holds the
   // longest rod; one element per thread up to 256 lanes, two elements per
-  // thread for 512 and 1024 lanes.  (FP32-bulk on ragged batches: 512 lanes at
-  // two elements per thread, two 4-wave workgroups per CU, run 27% faster than
-  // one element per thread in 8-wave workgroups; four elements per thread
-  // spill and run one wave per SIMD.)
-  // A shape the autotuner chose is taken again while it stays feasible (a
-  // replan after rods changed length keeps it, with its second group).
+  // thread for 512 and 1024 lanes; rods longer than 1024 elements run as
+  // pieces.  (FP32-bulk on ragged batches: 512 lanes at two elements per
+  // thread, two 4-wave workgroups per CU, run 27% faster than one element
+  // per thread in 8-wave workgroups; four elements per thread spill and run
+  // one wave per SIMD.)  A shape the autotuner chose is taken again while it
+  // stays feasible (a replan after rods changed length keeps it, with its
+  // second group).
   static int default_shape(const erod_batch* b) {
     static const int pref[] = {0, 1, 2, 4, 5, 3, 7, 6};
     if (b->plan.tuned_shape >= 0 && shape_feasible(b, b->plan.tuned_shape)) return b->plan.tuned_shape;
+    uint64_t maxn = 0;
+    for (uint64_t r = 0; r < b->n_rods; ++r)
+      if (planned(b, r)) maxn = std::max<uint64_t>(maxn, b->n_active[r]);
     for (int si : pref)
+      if (uint64_t(fused_shape_lanes(si)) >= maxn && shape_feasible(b, si)) return si;
+    // longer rods than any tile: the largest tiles, the longer rods in pieces
+    for (int si : {5, 4, 3, 2, 1, 0, 7, 6})
       if (shape_feasible(b, si)) return si;
     return -1;
   }
 
-  // One group of tiles on the host: the tiles, their rods in order and each
-  // rod's first lane; rods sent on to a second group.
+  // One group of tiles on the host: the tiles, their entries (rods or
+  // pieces) in order and each entry's first lane; rods sent on to a second group.
   struct Packed {
     std::vector<Tile> tiles;
     std::vector<uint32_t> trods, tlane0, overflow;
   };
 
-  // Best-fit-decreasing bin packing of the rods into tiles of P * NT elements:
-  // each rod, longest first, goes to the tile with the least free room that
+  // Best-fit-decreasing bin packing of entries into tiles of P * NT elements:
+  // each entry, longest first, goes to the tile with the least free room that
   // holds it (ties: the tile that reached that room first).  A workgroup costs
   // about the same whether its tile is full or not, so the tile count sets the
   // launch time; on the 4096-rod ragged batch this packs to the lower bound
   // (972 tiles of 1024 lanes, against 1170 for first fit over the 64 newest).
-  // Rod lengths and rooms are at most P * NT, so the sort is a counting sort
-  // and the open tiles sit in one FIFO list per room, found through a bitset:
-  // a serving pool replans its ~30,000 slots in about 0.5 ms of host time.
-  // Here `order` holds the rods, longest first, and cap is P * NT.  With
-  // spill_cap > 0, a rod of at most spill_cap elements that finds no room in an
-  // open tile goes to `overflow` instead of opening one, provided it has at
-  // least spill_min elements (the second group's elements per thread; a
-  // shorter rod stays in the main group).
-  static void pack(const erod_batch* b, uint64_t cap, const std::vector<uint32_t>& order, uint64_t spill_cap,
-                   uint64_t spill_min, Packed& out) {
+  // Lengths and rooms are at most P * NT, so the sort is a counting sort and
+  // the open tiles sit in one FIFO list per room, found through a bitset: a
+  // serving pool replans its ~30,000 slots in about 0.5 ms of host time.
+  // Here `order` holds the entries, longest first, len[e] their lengths, and
+  // cap is P * NT.  With spill_cap > 0, an entry of at most spill_cap
+  // elements that finds no room in an open tile goes to `overflow` instead of
+  // opening one, provided it has at least spill_min elements (the second
+  // group's elements per thread; a shorter rod stays in the main group).
+  static void pack(const std::vector<uint32_t>& len, uint64_t cap, const std::vector<uint32_t>& order,
+                   uint64_t spill_cap, uint64_t spill_min, Packed& out) {
     // open tiles by free room: FIFO lists (head, tail, next) and a bitset of non-empty rooms
     std::vector<int32_t> head(cap + 1, -1), tail(cap + 1, -1), next;
     std::vector<uint64_t> nonempty(cap / 64 + 1, 0);
@@ -155,7 +176,7 @@ namespace erod {
     };
     std::vector<uint32_t> placed, tile_of, tile_rods_count;
     for (const uint32_t r : order) {
-      const uint64_t n = b->n_active[r];
+      const uint64_t n = len[r];
       int64_t room = -1;  // the smallest room >= n that has a tile
       for (uint64_t w = n >> 6; w < nonempty.size() && room < 0; ++w) {
         uint64_t m = nonempty[w];
@@ -185,7 +206,7 @@ namespace erod {
       tile_of.push_back(t);
       ++tile_rods_count[t];
     }
-    // tiles in creation order, each with its rods in the order they were placed
+    // tiles in creation order, each with its entries in the order they were placed
     const size_t n_tiles = tile_rods_count.size(), n_placed = placed.size();
     out.tiles.assign(n_tiles, Tile{});
     out.trods.assign(n_placed, 0);
@@ -202,34 +223,79 @@ namespace erod {
       const uint32_t t = tile_of[i], k = fill[t]++;
       out.trods[k] = placed[i];
       out.tlane0[k] = lane[t];
-      lane[t] += static_cast<uint32_t>(b->n_active[placed[i]]);
+      lane[t] += len[placed[i]];
     }
     for (size_t t = 0; t < n_tiles; ++t) out.tiles[t].lanes = lane[t];
   }
 
-  // The device copy of one group.  The arrays grow to the largest plan seen
-  // (a tile or a rod entry per rod at most) and are reused when the rod set
-  // changes.
-  static int upload_group(erod_batch* b, const Packed& g, Tile*& d_tiles, uint32_t*& d_rods, uint32_t*& d_lane0,
-                          uint64_t& capacity) {
-    if (capacity < b->n_rods) {
-      EROD_HIP(hipStreamSynchronize(b->stream));
-      if (d_tiles) EROD_HIP(hipFree(d_tiles));
-      if (d_rods) EROD_HIP(hipFree(d_rods));
-      if (d_lane0) EROD_HIP(hipFree(d_lane0));
-      d_tiles = nullptr;
-      d_rods = d_lane0 = nullptr;
-      capacity = 0;
-      EROD_HIP(hipMalloc(&d_tiles, b->n_rods * sizeof(Tile)));
-      EROD_HIP(hipMalloc(&d_rods, b->n_rods * sizeof(uint32_t)));
-      EROD_HIP(hipMalloc(&d_lane0, b->n_rods * sizeof(uint32_t)));
-      capacity = b->n_rods;
+  // entries longest first (stable), by a counting sort over lengths <= cap
+  static std::vector<uint32_t> longest_first(const std::vector<uint32_t>& len, const std::vector<uint32_t>& ids,
+                                             uint64_t cap) {
+    std::vector<uint32_t> count(cap + 2, 0), next_pos(cap + 2, 0), order(ids.size());
+    for (const uint32_t e : ids) ++count[len[e]];
+    uint32_t k = 0;
+    for (uint64_t n = cap + 1; n >= 1; --n) {
+      next_pos[n] = k;
+      k += count[n];
     }
-    if (!g.tiles.empty()) {
-      EROD_HIP(hipMemcpyAsync(d_tiles, g.tiles.data(), g.tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, b->stream));
-      EROD_HIP(hipMemcpyAsync(d_rods, g.trods.data(), g.trods.size() * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
-      EROD_HIP(hipMemcpyAsync(d_lane0, g.tlane0.data(), g.tlane0.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+    for (const uint32_t e : ids) order[next_pos[len[e]]++] = e;
+    return order;
+  }
+
+  // The pieces of a rod of n elements for tiles of `cap` lanes: owned parts
+  // of nearly equal length, each piece loading `halo` more elements on each
+  // side that has a neighbouring piece; every piece at most cap long.
+  static void cut(uint32_t rod, uint64_t n, uint64_t cap, uint64_t halo, std::vector<Seg>& out) {
+    uint64_t m = 2;
+    while (true) {
+      // the longest piece of an even split into m: a middle part plus two halos
+      const uint64_t part = (n + m - 1) / m;
+      if (part + 2 * halo <= cap || part <= 1) break;
+      ++m;
+    }
+    for (uint64_t i = 0; i < m; ++i) {
+      const uint64_t a = i * n / m, z = (i + 1) * n / m;  // owned [a, z)
+      const uint64_t lo = i == 0 ? 0 : a - halo, hi = i + 1 == m ? n : z + halo;
+      Seg sg{};
+      sg.rod = rod;
+      sg.off = uint32_t(lo);
+      sg.n = uint32_t(hi - lo);
+      sg.w0 = uint32_t(a - lo);
+      sg.w1 = uint32_t(z - lo);
+      sg.ends = (i == 0 ? kSegFirst : 0u) | (i + 1 == m ? kSegLast : 0u);
+      out.push_back(sg);
+    }
+  }
+
+  // Device copy of one group.  The arrays grow to the largest plan seen and
+  // are reused when the rod set changes.
+  template <typename T>
+  static int grow(T*& p, uint64_t& cap, uint64_t need, hipStream_t stream) {
+    if (cap >= need) return EROD_OK;
+    EROD_HIP(hipStreamSynchronize(stream));
+    if (p) EROD_HIP(hipFree(p));
+    p = nullptr;
+    cap = 0;
+    EROD_HIP(hipMalloc(&p, need * sizeof(T)));
+    cap = need;
+    return EROD_OK;
+  }
+
+  static int upload_group(erod_batch* b, const Packed& pk, TileGroup& g) {
+    const uint64_t need = std::max<uint64_t>({pk.tiles.size(), pk.trods.size(), 1});
+    if (g.capacity < need) {
+      uint64_t c1 = g.capacity, c2 = g.capacity, c3 = g.capacity;
+      if (int rc = grow(g.d_tiles, c1, need, b->stream)) return rc;
+      if (int rc = grow(g.d_tile_rods, c2, need, b->stream)) return rc;
+      if (int rc = grow(g.d_tile_lane0, c3, need, b->stream)) return rc;
+      g.capacity = need;
+    }
+    if (!pk.tiles.empty()) {
+      EROD_HIP(hipMemcpyAsync(g.d_tiles, pk.tiles.data(), pk.tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, b->stream));
+      EROD_HIP(hipMemcpyAsync(g.d_tile_rods, pk.trods.data(), pk.trods.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
                               b->stream));
+      EROD_HIP(hipMemcpyAsync(g.d_tile_lane0, pk.tlane0.data(), pk.tlane0.size() * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, b->stream));
     }
     return EROD_OK;
   }
@@ -245,26 +311,36 @@ namespace erod {
   }
 
   static int plan_tiles(erod_batch* b, int si) {
-    const uint64_t cap = uint64_t(fused::kShapes[si].P) * fused::kShapes[si].NT;
-    // planned rods, longest first, stable
-    std::vector<uint32_t> count(cap + 1, 0);
+    const uint64_t cap = uint64_t(fused_shape_lanes(si));
+    const int ps = piece_shape(b, si);
+    if (ps == -2) {
+      set_error("plan_tiles: no tile shape of the requested piece lanes and elements per thread");
+      return EROD_ERR_INVALID;
+    }
+    const uint64_t pcap = uint64_t(fused_shape_lanes(ps));
+    const uint64_t halo = uint64_t(b->plan.halo);
+    if (halo < 2 || 2 * halo + 2 > pcap) {
+      set_error("plan_tiles: the halo of rod pieces must be at least 2 and leave room in the pieces' tile");
+      return EROD_ERR_INVALID;
+    }
+    // whole rods of the main (and second) group, and pieces of long rods
+    std::vector<uint32_t> len(b->n_rods), whole;
+    std::vector<uint32_t> seg_rods;
+    std::vector<Seg> segs;
     for (uint32_t i = 0; i < b->n_rods; ++i) {
       if (!planned(b, i)) continue;
-      if (b->n_active[i] > cap) {
-        set_error("plan_tiles: a rod is longer than the tile");
-        return EROD_ERR_INVALID;
+      len[i] = static_cast<uint32_t>(b->n_active[i]);
+      if (cut_rod(b, b->n_active[i], si)) {
+        if (b->n_active[i] > (uint64_t(1) << 31)) {
+          set_error("plan_tiles: a rod longer than 2^31 elements");
+          return EROD_ERR_INVALID;
+        }
+        seg_rods.push_back(i);
+        cut(i, b->n_active[i], pcap, halo, segs);
+      } else {
+        whole.push_back(i);
       }
-      ++count[b->n_active[i]];
     }
-    std::vector<uint32_t> next_pos(cap + 1, 0);
-    uint32_t n_planned = 0;
-    for (uint64_t n = cap; n >= 1; --n) {
-      next_pos[n] = n_planned;
-      n_planned += count[n];
-    }
-    std::vector<uint32_t> order(n_planned);
-    for (uint32_t i = 0; i < b->n_rods; ++i)
-      if (planned(b, i)) order[next_pos[b->n_active[i]]++] = i;
     const int s2 = split_shape(b);
     if (s2 == -2) {
       set_error("plan_tiles: no tile shape of the requested split lanes and elements per thread");
@@ -273,33 +349,56 @@ namespace erod {
     // a second group only of tiles smaller than the main ones (a requested
     // split at least as large as the main tile is ignored); it takes rods of
     // at most its lanes and at least its elements per thread
-    uint64_t cap2 = s2 >= 0 ? uint64_t(fused::kShapes[s2].P) * fused::kShapes[s2].NT : 0;
+    uint64_t cap2 = s2 >= 0 ? uint64_t(fused_shape_lanes(s2)) : 0;
     if (cap2 >= cap) cap2 = 0;
-    const uint64_t min2 = cap2 ? uint64_t(fused::kShapes[s2].P) : 0;
-    Packed g1, g2;
-    pack(b, cap, order, cap2, min2, g1);
-    if (!g1.overflow.empty()) pack(b, cap2, g1.overflow, 0, 0, g2);
-    // the kernel stages a tile's rod table in NT entries of LDS (rods have at
-    // least two elements, and at least P: a feasible shape never exceeds it)
-    for (const Tile& t : g1.tiles)
-      if (t.count > uint32_t(fused::kShapes[si].NT)) {
-        set_error("plan_tiles: more rods in a tile than threads");
-        return EROD_ERR_INVALID;
-      }
-    for (const Tile& t : g2.tiles)
-      if (t.count > uint32_t(fused::kShapes[s2].NT)) {
-        set_error("plan_tiles: more rods in a tile than threads");
-        return EROD_ERR_INVALID;
-      }
+    const uint64_t min2 = cap2 ? uint64_t(fused_shape_epl(s2)) : 0;
+    Packed g1, g2, g3;
+    pack(len, cap, longest_first(len, whole, cap), cap2, min2, g1);
+    if (!g1.overflow.empty()) pack(len, cap2, g1.overflow, 0, 0, g2);
+    std::vector<uint32_t> slen(segs.size()), sids(segs.size());
+    for (uint32_t k = 0; k < segs.size(); ++k) {
+      slen[k] = segs[k].n;
+      sids[k] = k;
+    }
+    if (!segs.empty()) pack(slen, pcap, longest_first(slen, sids, pcap), 0, 0, g3);
+    // the kernel stages a tile's entry table in NT entries of LDS (rods have
+    // at least two elements, and at least P: a feasible shape never exceeds it)
+    const int shapes[3] = {si, s2 >= 0 ? s2 : si, ps};
+    const Packed* pks[3] = {&g1, &g2, &g3};
+    for (int k = 0; k < 3; ++k)
+      for (const Tile& t : pks[k]->tiles)
+        if (t.count > uint32_t(fused::kShapes[shapes[k]].NT)) {
+          set_error("plan_tiles: more rods in a tile than threads");
+          return EROD_ERR_INVALID;
+        }
     FusedPlan& pl = b->plan;
-    if (int rc = upload_group(b, g1, pl.d_tiles, pl.d_tile_rods, pl.d_tile_lane0, pl.rod_capacity)) return rc;
-    if (!g2.tiles.empty())
-      if (int rc = upload_group(b, g2, pl.d_tiles2, pl.d_tile_rods2, pl.d_tile_lane02, pl.rod_capacity2)) return rc;
+    for (int k = 0; k < 3; ++k)
+      if (!pks[k]->tiles.empty())
+        if (int rc = upload_group(b, *pks[k], pl.g[k])) return rc;
+    TileGroup& gs = pl.g[2];
+    if (!segs.empty()) {
+      if (int rc = grow(gs.d_segs, gs.seg_capacity, segs.size(), b->stream)) return rc;
+      if (int rc = grow(gs.d_seg_rods, gs.seg_rod_capacity, seg_rods.size(), b->stream)) return rc;
+      if (!b->d_time_next) EROD_HIP(hipMalloc(&b->d_time_next, sizeof(double) * b->n_rods));
+      const int shadow_fields[4] = {EROD_POSITION, EROD_VELOCITY, EROD_DIRECTOR, EROD_ANGULAR_VELOCITY};
+      for (int k = 0; k < 4; ++k)
+        if (!b->d_shadow[k])
+          EROD_HIP(hipMalloc(&b->d_shadow[k], sizeof(double) * field_info(shadow_fields[k]).ncomp *
+                                                  b->field_slots(shadow_fields[k])));
+      EROD_HIP(hipMemcpyAsync(gs.d_segs, segs.data(), segs.size() * sizeof(Seg), hipMemcpyHostToDevice, b->stream));
+      EROD_HIP(hipMemcpyAsync(gs.d_seg_rods, seg_rods.data(), seg_rods.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              b->stream));
+    }
     EROD_HIP(hipStreamSynchronize(b->stream));
-    pl.n_tiles = g1.tiles.size();
-    pl.shape = si;
-    pl.n_tiles2 = g2.tiles.size();
-    pl.shape2 = g2.tiles.empty() ? -1 : s2;
+    for (int k = 0; k < 3; ++k) {
+      pl.g[k].n_tiles = pks[k]->tiles.size();
+      pl.g[k].shape = pks[k]->tiles.empty() ? -1 : shapes[k];
+      pl.g[k].max_steps = 0;
+    }
+    gs.n_segs = segs.size();
+    gs.n_seg_rods = seg_rods.size();
+    gs.max_steps = int(halo / 2);
+    if (segs.empty()) gs.shape = -1;
     pl.valid = true;
     b->graph_precision = -1;
     return EROD_OK;
@@ -311,8 +410,8 @@ namespace erod {
     if (!b->plan.valid) {
       const int si = default_shape(b);
       if (si < 0) {
-        set_error("no fused tile shape holds this batch (rods longer than 1024 elements, or a requested shape "
-                  "that does not fit)");
+        set_error("no fused tile shape holds this batch (a requested shape with more elements per thread than "
+                  "a rod has)");
         return EROD_ERR_UNSUPPORTED;
       }
       if ((rc = plan_tiles(b, si))) return rc;
@@ -353,6 +452,8 @@ namespace erod {
     C.D0 = b->d_field[EROD_REFERENCE_VORONOI_LENGTH];
     C.kap0 = b->d_field[EROD_REFERENCE_CURVATURE];
     C.B = b->d_field[EROD_BENDING_TWIST_RIGIDITY];
+    C.gf = b->d_field[EROD_FORCE_DAMPING_RATE];
+    C.gt = b->d_field[EROD_TORQUE_DAMPING_RATE];
     return C;
   }
 
@@ -370,6 +471,8 @@ namespace erod {
     C.D0 = p + const32_offset(b, 7);
     C.kap0 = p + const32_offset(b, 8);
     C.B = p + const32_offset(b, 9);
+    C.gf = p + const32_offset(b, 10);
+    C.gt = p + const32_offset(b, 11);
     return C;
   }
 
@@ -447,7 +550,7 @@ namespace erod {
           rc = prc;
           break;
         }
-        if (prc != EROD_OK || b->plan.shape2 < 0) continue;  // infeasible here, or no second group
+        if (prc != EROD_OK || b->plan.g[1].shape < 0) continue;  // infeasible here, or no second group
         if ((rc = fused_launch(b, precision, trial_steps, false))) break;  // warm
         if ((rc = fused_launch(b, precision, trial_steps, true))) break;
         float ms = 0.0f;
@@ -499,8 +602,9 @@ namespace erod {
     if (rc) return rc;
     EROD_HIP(hipMemsetAsync(b->d_flags, 0, sizeof(int) * b->n_rods, b->stream));
     const unsigned g = static_cast<unsigned>(b->n_rods < 65535 ? b->n_rods : 65535);
-    hipLaunchKernelGGL(k_check_finite, dim3(g), dim3(256), 0, b->stream, b->d_field[EROD_POSITION], b->d_field[EROD_VELOCITY],
-                       b->d_field[EROD_DIRECTOR], b->d_field[EROD_ANGULAR_VELOCITY], b->d_rods, b->n_rods, b->E, b->d_flags);
+    hipLaunchKernelGGL(k_check_finite<int>, dim3(g), dim3(256), 0, b->stream, b->d_field[EROD_POSITION],
+                       b->d_field[EROD_VELOCITY], b->d_field[EROD_DIRECTOR], b->d_field[EROD_ANGULAR_VELOCITY], b->d_rods,
+                       b->n_rods, b->E, b->d_flags);
     EROD_HIP(hipGetLastError());
     std::vector<int> h(b->n_rods);
     EROD_HIP(hipMemcpyAsync(h.data(), b->d_flags, sizeof(int) * b->n_rods, hipMemcpyDeviceToHost, b->stream));

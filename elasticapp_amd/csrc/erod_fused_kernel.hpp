@@ -63,6 +63,19 @@ namespace erod {
       const R* D0;    // voronoi
       const R* kap0;  // voronoi, 3 lines
       const R* B;     // voronoi, 3 lines
+      const R* gf;    // node: force damping rate (read for rods whose rates vary along them)
+      const R* gt;    // element: torque damping rate (same)
+    };
+
+    // the state arrays a launch reads (in) and writes (out): the batch's
+    // fields for whole rods; for pieces of rods, which read their halo from
+    // neighbouring pieces, alternately the fields and a second copy, so that
+    // no launch reads what it writes
+    struct StatePtrs {
+      double* x;  // node, 3 lines
+      double* v;  // node, 3 lines
+      double* q;  // element, 9 lines
+      double* w;  // element, 3 lines
     };
 
     // ---- small helpers ---------------------------------------------------------
@@ -480,12 +493,22 @@ namespace erod {
 
     // kEnv: 0 no environment, 1 gravity and / or contact with a level plane,
     // 2 contact with a plane of any normal
+    //
+    // The entries of a tile are whole rods, or, with `segs`, pieces of rods
+    // (Seg): a piece runs as a rod of its own length whose elements outside
+    // [w0, w1) are not written back, and whose cut ends take the rod-end
+    // rules with neither clamp nor end force (its halo absorbs the error).
+    // Rod clocks are read from time_in and written to time_out, step budgets
+    // read from steps_in and, when steps_out is set, written there: for whole
+    // rods the same arrays; over pieces, which share their rod's clock and
+    // budget across tiles, the clock goes to a second array (from the piece
+    // holding the rod's last element) and the budget is spent after the launch.
     template <typename R, int P, int NT, int kEnv>
-    __global__ void __launch_bounds__(NT) k_fused(double* __restrict__ X, double* __restrict__ Vel, double* __restrict__ Qd,
-                                                  double* __restrict__ W, Consts<R> C, const RodInfo* __restrict__ rods,
+    __global__ void __launch_bounds__(NT) k_fused(StatePtrs in, StatePtrs out, Consts<R> C, const RodInfo* __restrict__ rods,
                                                   const Tile* __restrict__ tiles, const uint32_t* __restrict__ tile_rods,
-                                                  const uint32_t* __restrict__ tile_lane0, double* __restrict__ rod_time,
-                                                  uint32_t* __restrict__ steps_left, Env env, int nsteps, uint64_t E) {
+                                                  const uint32_t* __restrict__ tile_lane0, const Seg* __restrict__ segs,
+                                                  const double* time_in, double* time_out, const uint32_t* steps_in,
+                                                  uint32_t* steps_out, Env env, int nsteps, uint64_t E) {
       constexpr bool kCache = sizeof(R) == 4;  // FP32 constants stay in registers for the launch
       // FP32-bulk: the closing half step of one step and the opening half step
       // of the next (same v and w) are taken as one update of dt within a launch
@@ -521,8 +544,11 @@ namespace erod {
       double tm[P], dt[P];
       uint32_t rflags[P];
       uint32_t lim[P];  // steps this launch may take for the element's rod (its step budget)
+      bool own[P];      // written back (false in a piece's halo)
+      bool clockw[P];   // writes its rod's clock and budget
+      bool tailw = false;  // the thread's tail node is its rod's last node (written back)
       R gfj[P], gtj[P];
-      R tail_m = R(0), tail_im = R(0);
+      R tail_m = R(0), tail_im = R(0), tail_gf = R(0);
       double xt[3] = {0, 0, 0}, vt[3] = {0, 0, 0};
       int tailj = -1;
       // FP32-bulk without environment: the end force on the rod's last node
@@ -559,18 +585,41 @@ namespace erod {
         const uint32_t el = uint32_t(tid) * P + j;
         valid[j] = el < tile.lanes;
         const uint32_t e = find(el);
-        const uint32_t r = t_rod[e];
+        const uint32_t ent = t_rod[e];
+        uint32_t r, off, segn, w0, w1, ends;
+        if (segs) {
+          const Seg sg = segs[ent];
+          r = sg.rod;
+          off = sg.off;
+          segn = sg.n;
+          w0 = sg.w0;
+          w1 = sg.w1;
+          ends = sg.ends;
+        } else {
+          r = ent;
+          off = 0;
+          segn = rods[r].n;
+          w0 = 0;
+          w1 = segn;
+          ends = kSegFirst | kSegLast;
+        }
         rodi[j] = int(r);
         const RodInfo& ri = rods[r];
         kk[j] = valid[j] ? int(el - t_lane0[e]) : 0;
-        nn[j] = int(ri.n);
-        slot[j] = ri.start + uint64_t(kk[j]);
+        nn[j] = int(segn);
+        slot[j] = ri.start + uint64_t(off) + uint64_t(kk[j]);
         dt[j] = ri.dt;
-        tm[j] = rod_time[r];
-        lim[j] = steps_left ? steps_left[r] : kNoBudget;
-        rflags[j] = ri.flags;
-        gfj[j] = static_cast<R>(ri.gamma_f);
-        gtj[j] = static_cast<R>(ri.gamma_t);
+        tm[j] = time_in[r];
+        lim[j] = steps_in ? steps_in[r] : kNoBudget;
+        // the clamp acts at the rod's first element, the end force at its last node
+        rflags[j] = ri.flags & ((ends & kSegFirst) ? ~0u : ~1u) & ((ends & kSegLast) ? ~0u : ~2u);
+        own[j] = valid[j] && uint32_t(kk[j]) >= w0 && uint32_t(kk[j]) < w1;
+        clockw[j] = valid[j] && kk[j] == 0 && (segs == nullptr || (ends & kSegLast) != 0);
+        if (valid[j] && kk[j] == int(segn) - 1) tailw = (ends & kSegLast) != 0;
+        // damping rates: one per rod, or (flag bit 2) per node and element
+        const bool per_node = (ri.flags & 4u) != 0 && valid[j];
+        gfj[j] = per_node ? C.gf[slot[j]] : static_cast<R>(ri.gamma_f);
+        gtj[j] = per_node ? C.gt[slot[j]] : static_cast<R>(ri.gamma_t);
       }
       // pass 2: the state, one field line at a time over the P elements (their
       // slots are contiguous within a rod, so the P loads of a line share cache
@@ -578,22 +627,22 @@ namespace erod {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
 #pragma unroll
-        for (int j = 0; j < P; ++j) x[j][c] = valid[j] ? X[c * N + slot[j]] : 0.0;
+        for (int j = 0; j < P; ++j) x[j][c] = valid[j] ? in.x[c * N + slot[j]] : 0.0;
       }
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
 #pragma unroll
-        for (int j = 0; j < P; ++j) v[j][c] = valid[j] ? Vel[c * N + slot[j]] : 0.0;
+        for (int j = 0; j < P; ++j) v[j][c] = valid[j] ? in.v[c * N + slot[j]] : 0.0;
       }
 #pragma unroll
       for (int m = 0; m < 9; ++m) {
 #pragma unroll
-        for (int j = 0; j < P; ++j) q[j][m] = valid[j] ? Qd[m * E + slot[j]] : ((m % 4 == 0) ? 1.0 : 0.0);
+        for (int j = 0; j < P; ++j) q[j][m] = valid[j] ? in.q[m * E + slot[j]] : ((m % 4 == 0) ? 1.0 : 0.0);
       }
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
 #pragma unroll
-        for (int j = 0; j < P; ++j) w[j][c] = valid[j] ? W[c * E + slot[j]] : 0.0;
+        for (int j = 0; j < P; ++j) w[j][c] = valid[j] ? in.w[c * E + slot[j]] : 0.0;
       }
 #pragma unroll
       for (int j = 0; j < P; ++j) {
@@ -601,11 +650,12 @@ namespace erod {
           tailj = j;
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            xt[c] = X[c * N + slot[j] + 1];
-            vt[c] = Vel[c * N + slot[j] + 1];
+            xt[c] = in.x[c * N + slot[j] + 1];
+            vt[c] = in.v[c * N + slot[j] + 1];
           }
           tail_m = C.m[slot[j] + 1];
           tail_im = C.invm[slot[j] + 1];
+          tail_gf = (rods[rodi[j]].flags & 4u) ? C.gf[slot[j] + 1] : gfj[j];
           if (kTailForce && (rflags[j] & 2u)) {
             const RodInfo& ri = rods[rodi[j]];
 #pragma unroll
@@ -871,7 +921,7 @@ namespace erod {
           }
           // last node of the rod
           if (last) {
-            const R mgn = tail_m * gf;
+            const R mgn = tail_m * tail_gf;
             double fen[3] = {0.0, 0.0, 0.0};
             if (rflags[j] & 2u) {
               if constexpr (kTailForce) {
@@ -929,72 +979,116 @@ namespace erod {
       for (int c = 0; c < 3; ++c) {
 #pragma unroll
         for (int j = 0; j < P; ++j)
-          if (valid[j]) X[c * N + slot[j]] = x[j][c];
+          if (own[j]) out.x[c * N + slot[j]] = x[j][c];
       }
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
 #pragma unroll
         for (int j = 0; j < P; ++j)
-          if (valid[j]) Vel[c * N + slot[j]] = v[j][c];
+          if (own[j]) out.v[c * N + slot[j]] = v[j][c];
       }
 #pragma unroll
       for (int m = 0; m < 9; ++m) {
 #pragma unroll
         for (int j = 0; j < P; ++j)
-          if (valid[j]) Qd[m * E + slot[j]] = q[j][m];
+          if (own[j]) out.q[m * E + slot[j]] = q[j][m];
       }
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
 #pragma unroll
         for (int j = 0; j < P; ++j)
-          if (valid[j]) W[c * E + slot[j]] = w[j][c];
+          if (own[j]) out.w[c * E + slot[j]] = w[j][c];
       }
 #pragma unroll
       for (int j = 0; j < P; ++j) {
         if (!valid[j]) continue;
         const uint64_t s = slot[j];
-        if (j == tailj) {
+        if (j == tailj && tailw) {
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            X[c * N + s + 1] = xt[c];
-            Vel[c * N + s + 1] = vt[c];
+            out.x[c * N + s + 1] = xt[c];
+            out.v[c * N + s + 1] = vt[c];
           }
         }
-        if (kk[j] == 0) {
-          rod_time[rodi[j]] = tm[j];
-          if (steps_left && lim[j] != kNoBudget) steps_left[rodi[j]] = lim[j] > uint32_t(nsteps) ? lim[j] - uint32_t(nsteps) : 0u;
+        if (clockw[j]) {
+          time_out[rodi[j]] = tm[j];
+          if (steps_out && lim[j] != kNoBudget) steps_out[rodi[j]] = lim[j] > uint32_t(nsteps) ? lim[j] - uint32_t(nsteps) : 0u;
         }
       }
     }
 
-    // the tiles of one group of a plan
-    struct Group {
-      uint64_t n_tiles;
-      const Tile* tiles;
-      const uint32_t* rods;
-      const uint32_t* lane0;
-    };
+    // the state of the rods cut into pieces, from the second copy back into
+    // the fields (after an odd number of launches over pieces); one
+    // workgroup per rod
+    static __global__ void k_copy_cut_state(const uint32_t* __restrict__ seg_rods, const RodInfo* __restrict__ rods,
+                                            StatePtrs from, StatePtrs to, uint64_t E) {
+      const uint64_t N = E + 1;
+      const RodInfo ri = rods[seg_rods[blockIdx.x]];
+      for (uint32_t k = threadIdx.x; k <= ri.n; k += blockDim.x) {
+        const uint64_t s = ri.start + k;
+        for (int c = 0; c < 3; ++c) {
+          to.x[c * N + s] = from.x[c * N + s];
+          to.v[c * N + s] = from.v[c * N + s];
+        }
+        if (k < ri.n) {
+          for (int m = 0; m < 9; ++m) to.q[m * E + s] = from.q[m * E + s];
+          for (int c = 0; c < 3; ++c) to.w[c * E + s] = from.w[c * E + s];
+        }
+      }
+    }
+
+    // after a launch over pieces: the rods' clocks (written by the pieces
+    // holding their last element) and budgets (spent by the launch's steps)
+    static __global__ void k_seg_clocks(const uint32_t* __restrict__ seg_rods, uint64_t n, const double* __restrict__ time_next,
+                                 double* __restrict__ rod_time, uint32_t* __restrict__ steps_left, uint32_t nsteps) {
+      const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+      if (i >= n) return;
+      const uint32_t r = seg_rods[i];
+      rod_time[r] = time_next[r];
+      if (steps_left) {
+        const uint32_t l = steps_left[r];
+        if (l != kNoBudget) steps_left[r] = l > nsteps ? l - nsteps : 0u;
+      }
+    }
+
+    inline StatePtrs field_state(erod_batch* b) {
+      return StatePtrs{b->d_field[EROD_POSITION], b->d_field[EROD_VELOCITY], b->d_field[EROD_DIRECTOR],
+                       b->d_field[EROD_ANGULAR_VELOCITY]};
+    }
+    inline StatePtrs shadow_state(erod_batch* b) {
+      return StatePtrs{b->d_shadow[0], b->d_shadow[1], b->d_shadow[2], b->d_shadow[3]};
+    }
 
     template <typename R, int P, int NT, int kEnv>
-    inline void launch_env(erod_batch* b, const Consts<R>& C, int steps, const Group& g) {
+    inline void launch_env(erod_batch* b, const Consts<R>& C, int steps, const TileGroup& g, const StatePtrs& in,
+                           const StatePtrs& out) {
       uint32_t* budgets = b->budgets ? b->d_steps_left : nullptr;
-      hipLaunchKernelGGL((k_fused<R, P, NT, kEnv>), dim3(static_cast<unsigned>(g.n_tiles)), dim3(NT), 0, b->stream,
-                         b->d_field[EROD_POSITION], b->d_field[EROD_VELOCITY], b->d_field[EROD_DIRECTOR],
-                         b->d_field[EROD_ANGULAR_VELOCITY], C, b->d_rods, g.tiles, g.rods, g.lane0, b->d_time, budgets,
-                         b->env, steps, b->E);
+      if (g.d_segs) {
+        // pieces: clocks to the second array, budgets spent afterwards
+        hipLaunchKernelGGL((k_fused<R, P, NT, kEnv>), dim3(static_cast<unsigned>(g.n_tiles)), dim3(NT), 0, b->stream, in,
+                           out, C, b->d_rods, g.d_tiles, g.d_tile_rods, g.d_tile_lane0, g.d_segs, b->d_time, b->d_time_next,
+                           budgets, nullptr, b->env, steps, b->E);
+        hipLaunchKernelGGL(k_seg_clocks, dim3(static_cast<unsigned>((g.n_seg_rods + 255) / 256)), dim3(256), 0, b->stream,
+                           g.d_seg_rods, g.n_seg_rods, b->d_time_next, b->d_time, budgets, uint32_t(steps));
+      } else {
+        hipLaunchKernelGGL((k_fused<R, P, NT, kEnv>), dim3(static_cast<unsigned>(g.n_tiles)), dim3(NT), 0, b->stream, in,
+                           out, C, b->d_rods, g.d_tiles, g.d_tile_rods, g.d_tile_lane0, nullptr, b->d_time, b->d_time,
+                           budgets, budgets, b->env, steps, b->E);
+      }
     }
 
     // batches without gravity or contact run a kernel without environment code
     // (and without the registers its parameters would hold); a level contact
     // plane runs one without the arithmetic of a general normal
     template <typename R, int P, int NT>
-    inline int launch_tiles(erod_batch* b, const Consts<R>& C, int steps, const Group& g) {
+    inline int launch_tiles(erod_batch* b, const Consts<R>& C, int steps, const TileGroup& g, const StatePtrs& in,
+                            const StatePtrs& out) {
       if (g.n_tiles == 0) return EROD_OK;
       const Env& e = b->env;
       const bool level = e.n[0] == 0.0 && e.n[1] == 0.0 && e.n[2] == 1.0;
-      if (e.contact && !level) launch_env<R, P, NT, 2>(b, C, steps, g);
-      else if (e.gravity || e.contact) launch_env<R, P, NT, 1>(b, C, steps, g);
-      else launch_env<R, P, NT, 0>(b, C, steps, g);
+      if (e.contact && !level) launch_env<R, P, NT, 2>(b, C, steps, g, in, out);
+      else if (e.gravity || e.contact) launch_env<R, P, NT, 1>(b, C, steps, g, in, out);
+      else launch_env<R, P, NT, 0>(b, C, steps, g, in, out);
       return hip_check(hipGetLastError(), "k_fused");
     }
 
@@ -1006,30 +1100,56 @@ namespace erod {
     constexpr int kNShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
     template <typename R>
-    inline int launch_group(erod_batch* b, const Consts<R>& C, int steps, int shape, const Group& g) {
+    inline int launch_shape(erod_batch* b, const Consts<R>& C, int steps, int shape, const TileGroup& g,
+                            const StatePtrs& in, const StatePtrs& out) {
       switch (shape) {
-        case 0: return launch_tiles<R, 1, 64>(b, C, steps, g);
-        case 1: return launch_tiles<R, 1, 128>(b, C, steps, g);
-        case 2: return launch_tiles<R, 1, 256>(b, C, steps, g);
-        case 3: return launch_tiles<R, 1, 512>(b, C, steps, g);
-        case 4: return launch_tiles<R, 2, 256>(b, C, steps, g);
-        case 5: return launch_tiles<R, 2, 512>(b, C, steps, g);
-        case 6: return launch_tiles<R, 4, 128>(b, C, steps, g);
-        case 7: return launch_tiles<R, 4, 256>(b, C, steps, g);
+        case 0: return launch_tiles<R, 1, 64>(b, C, steps, g, in, out);
+        case 1: return launch_tiles<R, 1, 128>(b, C, steps, g, in, out);
+        case 2: return launch_tiles<R, 1, 256>(b, C, steps, g, in, out);
+        case 3: return launch_tiles<R, 1, 512>(b, C, steps, g, in, out);
+        case 4: return launch_tiles<R, 2, 256>(b, C, steps, g, in, out);
+        case 5: return launch_tiles<R, 2, 512>(b, C, steps, g, in, out);
+        case 6: return launch_tiles<R, 4, 128>(b, C, steps, g, in, out);
+        case 7: return launch_tiles<R, 4, 256>(b, C, steps, g, in, out);
         default: set_error("bad tile shape"); return EROD_ERR_INVALID;
       }
     }
 
-    // one launch of `steps` steps over every tile of the plan: its main group,
-    // then its second group when it has one (rods never interact, so the
-    // order does not matter)
+    // one group of tiles advanced `steps` steps: one launch, or, for a group
+    // with a limit on its steps per launch (pieces of rods), launches of at
+    // most that many
+    template <typename R>
+    inline int launch_group(erod_batch* b, const Consts<R>& C, int steps, const TileGroup& g) {
+      if (g.shape < 0 || g.n_tiles == 0) return EROD_OK;
+      const StatePtrs fields = field_state(b);
+      if (!g.d_segs) return launch_shape(b, C, steps, g.shape, g, fields, fields);
+      // pieces: launches of at most max_steps steps, each reading the state
+      // the previous one wrote and writing the other copy; the state ends in
+      // the fields
+      const StatePtrs shadow = shadow_state(b);
+      const int cap = g.max_steps > 0 ? g.max_steps : steps;
+      bool in_fields = true;
+      for (int done = 0; done < steps;) {
+        const int s = steps - done < cap ? steps - done : cap;
+        if (int rc = launch_shape(b, C, s, g.shape, g, in_fields ? fields : shadow, in_fields ? shadow : fields)) return rc;
+        in_fields = !in_fields;
+        done += s;
+      }
+      if (!in_fields) {
+        hipLaunchKernelGGL(k_copy_cut_state, dim3(static_cast<unsigned>(g.n_seg_rods)), dim3(256), 0, b->stream, g.d_seg_rods,
+                           b->d_rods, shadow, fields, b->E);
+        return hip_check(hipGetLastError(), "k_copy_cut_state");
+      }
+      return EROD_OK;
+    }
+
+    // `steps` steps over every tile of the plan: its main group, its second
+    // group, the pieces of its long rods (rods never interact, so the order
+    // does not matter)
     template <typename R>
     inline int launch_all(erod_batch* b, const Consts<R>& C, int steps) {
-      const FusedPlan& pl = b->plan;
-      if (int rc = launch_group(b, C, steps, pl.shape, Group{pl.n_tiles, pl.d_tiles, pl.d_tile_rods, pl.d_tile_lane0}))
-        return rc;
-      if (pl.shape2 >= 0 && pl.n_tiles2 > 0)
-        return launch_group(b, C, steps, pl.shape2, Group{pl.n_tiles2, pl.d_tiles2, pl.d_tile_rods2, pl.d_tile_lane02});
+      for (const TileGroup& g : b->plan.g)
+        if (int rc = launch_group(b, C, steps, g)) return rc;
       return EROD_OK;
     }
 

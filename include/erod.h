@@ -163,7 +163,8 @@ int erod_set_rod_time(erod_batch* b, const double* time_per_rod);
  * steps_per_launch: steps a workgroup advances with its rods' state held on
  * chip (0 = 100).  The rods of a batch share one shape.  Without a request or
  * an autotune, a batch takes the smallest capacity that holds its longest rod
- * (one element per thread up to 256 lanes, two from 512). */
+ * (one element per thread up to 256 lanes, two from 512); rods longer than
+ * the tile run as pieces (erod_set_fused_pieces). */
 int erod_set_fused_config(erod_batch* b, int lanes_per_tile, int elements_per_thread, int steps_per_launch);
 int erod_get_fused_config(const erod_batch* b, int* lanes_per_tile, int* elements_per_thread, int* steps_per_launch);
 /* A second group of tiles: a rod of at most lanes_per_tile elements that finds
@@ -175,6 +176,20 @@ int erod_get_fused_config(const erod_batch* b, int* lanes_per_tile, int* element
  * reports the group's shape and tile count of the current plan (0 if none). */
 int erod_set_fused_split(erod_batch* b, int lanes_per_tile, int elements_per_thread);
 int erod_get_fused_split(const erod_batch* b, int* lanes_per_tile, int* elements_per_thread, uint64_t* tiles);
+/* Rods longer than the main tile, or longer than lanes_per_tile when it is
+ * not 0, run as pieces in tiles of that shape (0: the main shape;
+ * elements_per_thread 0 = either): each piece owns a part of its rod and
+ * recomputes `halo` elements (0 = 16, at least 2) of each neighbouring piece
+ * from the state they had when the launch began, so that pieces need no
+ * exchange within a launch; a launch over pieces then takes at most halo / 2
+ * steps, and the pieces' group is launched as often as needed.  Results do
+ * not depend on the cut (bitwise).  For rods longer than any tile (the
+ * reference's blocks emplace rods of any length, Block.hpp:568-611) and for
+ * fused FP64, whose long-rod tiles spill registers.  The getter reports the
+ * pieces' shape, halo, tiles and pieces in the current plan (0 if none). */
+int erod_set_fused_pieces(erod_batch* b, int lanes_per_tile, int elements_per_thread, int halo);
+int erod_get_fused_pieces(const erod_batch* b, int* lanes_per_tile, int* elements_per_thread, int* halo,
+                          uint64_t* tiles, uint64_t* pieces);
 /* Times every feasible tile shape on `trial_steps` steps from a snapshot of
  * the state (restored afterwards) and keeps the fastest; reports its ms/step.
  * No counterpart in the reference (its blocks are sized by a fixed policy,
@@ -188,6 +203,10 @@ int erod_step_fused(erod_batch* b, int precision, uint64_t n);
  * launching the n % chunk steps left directly (in launches of
  * steps_per_launch, the last one shorter). */
 int erod_step_fused_graph(erod_batch* b, int precision, uint64_t n, uint64_t chunk);
+/* Capture the hipGraph of one chunk of `chunk` fused steps now (as
+ * erod_step_fused_graph would on its first replay), without launching it, so
+ * that a later erod_step_fused_graph of that chunk starts replaying at once. */
+int erod_capture_fused_graph(erod_batch* b, int precision, uint64_t chunk);
 /* Re-orthonormalise every director (one Newton-Schulz polar step, FP64). */
 int erod_reorthonormalize(erod_batch* b);
 /* Average duration (ms) of one launch of the fused step kernel over the last

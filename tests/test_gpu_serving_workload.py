@@ -6,9 +6,12 @@ materials, tip forces and each rod's own stable dt; `serving_requests`: 1-64
 rods per request), with horizons cut to 1,000-2,000 steps so the test fits the
 suite.  Requests are admitted in waves while earlier ones run.  For a sample
 of completed requests every rod is compared with the oracle stepping the
-same rod alone at its own dt (FP64 server: positions, directors and
-velocities to 1e-10 relative; FP32-bulk: positions to 1e-6 of the rod
-length), and with the same request served alone (bitwise).  The nearest
+same rod alone at its own dt (FP64 server: bitwise against the oracle built
+with the device library's sin / cos / acos, and within 1e-7 of the
+C-library oracle, the bound the spread of those functions' last bits keeps
+on the 1024 x 64 batch over 10,000 steps (tests/test_oracle_conditioning.py);
+FP32-bulk: positions to 1e-6 of the rod length), and with the same
+request served alone (bitwise).  The nearest
 reference analogue of admission is first-fit blocking of rods at emplace
 time (elastica/Simulator/Configuration/ModuleConfiguration/BlockingPolicy.hpp:202-222).
 """
@@ -36,12 +39,12 @@ def library():
     lib.close()
 
 
-def _oracle(wl, dts, rods, steps):
+def _oracle(wl, dts, rods, steps, variant=None):
     """The request's rods, each in a block of its own, stepped `steps` steps
     at its own dt by the oracle."""
     sub = Workload("sub", [wl.n_elems[r] for r in rods], [wl.states[r] for r in rods], wl.dt, 0,
                    forces=[(i,) + tuple(wl.forces[r][1:]) for i, r in enumerate(rods)])
-    sysm = oracle_system(sub, max_block_elems=0)
+    sysm = oracle_system(sub, max_block_elems=0, variant=variant)
     sysm.step_per_block_dt(np.array([dts[r] for r in rods]), steps, min(16, len(os.sched_getaffinity(0))))
     return sysm
 
@@ -85,14 +88,16 @@ def test_served_requests_match_the_oracle(library, precision):
     for rid in sample:
         rods, steps = reqs[rid]
         sysm = _oracle(wl, dts, list(rods), steps)
+        dev = _oracle(wl, dts, list(rods), steps, variant="devmath") if precision == FP64 else None
         for k, r in enumerate(rods):
             L = float(wl.states[r]["ReferenceElementLength"].sum())
             for t in STATE:
                 c = sysm.rod_field(k, t)
                 g = results[rid][k][t]
                 if precision == FP64:
+                    assert np.array_equal(g, dev.rod_field(k, t)), (rid, k, t)
                     e = float(np.abs(g - c).max() / max(np.abs(c).max(), 1e-300))
-                    assert e <= 1e-10, (rid, k, t, e)
+                    assert e <= 1e-7, (rid, k, t, e)
                 elif t == "Position":
                     e = float(np.abs(g - c).max() / L)
                     assert e <= 1e-6, (rid, k, t, e)
@@ -100,7 +105,9 @@ def test_served_requests_match_the_oracle(library, precision):
                     continue
                 worst[t] = max(worst.get(t, 0.0), e)
         sysm.close()
-    print("worst error against the oracle over the sample:", worst, flush=True)
+        if dev is not None:
+            dev.close()
+    print("worst error against the C-library oracle over the sample:", worst, flush=True)
 
 
 def test_served_request_alone_equals_in_the_mix(library):
