@@ -106,6 +106,7 @@ def load_library():
         "erod_last_kernel_ms": (i32, [vp, P(dbl), P(u64)]),
         "erod_synchronize": (i32, [vp]),
         "erod_check_finite": (i32, [vp, P(ctypes.c_uint8)]),
+        "erod_check_finite_noalloc": (i32, [vp, P(ctypes.c_uint8)]),
         "erod_save": (i32, [vp, ctypes.c_char_p]),
         "erod_load": (i32, [ctypes.c_char_p, P(vp)]),
         "erod_set_rod_active_length": (i32, [vp, u64, u64]),

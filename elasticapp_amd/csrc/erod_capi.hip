@@ -844,4 +844,9 @@ int erod_check_finite(erod_batch* b, uint8_t* rod_flags) {
   return fused_check_finite(b, rod_flags);
 }
 
+int erod_check_finite_noalloc(erod_batch* b, uint8_t* rod_flags) {
+  if (check_batch(b) || !rod_flags) return EROD_ERR_INVALID;
+  return fused_check_finite_noalloc(b, rod_flags);
+}
+
 }  // extern "C"

@@ -232,6 +232,7 @@ namespace erod {
   int fused_launch(erod_batch* b, int precision, int steps, bool timed);
   int fused_reorthonormalize(erod_batch* b);
   int fused_check_finite(erod_batch* b, uint8_t* flags);
+  int fused_check_finite_noalloc(erod_batch* b, uint8_t* flags);
   int fused_autotune(erod_batch* b, int precision, int trial_steps, double* best_ms);
   // offset (in floats) of a field in the FP32 constant copy, or ~0 if the field
   // has none; valid while the copy is allocated and not dirty

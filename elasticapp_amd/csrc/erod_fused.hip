@@ -2,6 +2,7 @@
 // the FP32 copies of the constants, launches, and the director
 // re-orthonormalisation and finiteness checks.
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "erod_fused_kernel.hpp"
@@ -74,6 +75,11 @@ namespace erod {
         }
         if (bad) flags[r] = Flag(1);
       }
+    }
+
+    __global__ void k_zero_u8(uint8_t* __restrict__ p, uint64_t n) {
+      for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
     }
 
   }  // namespace
@@ -610,6 +616,72 @@ namespace erod {
     EROD_HIP(hipMemcpyAsync(h.data(), b->d_flags, sizeof(int) * b->n_rods, hipMemcpyDeviceToHost, b->stream));
     EROD_HIP(hipStreamSynchronize(b->stream));
     for (uint64_t i = 0; i < b->n_rods; ++i) flags[i] = static_cast<uint8_t>(h[i] != 0);
+    return EROD_OK;
+  }
+
+  // ---- finiteness check that allocates nothing -----------------------------------
+  //
+  // The per-rod flags live in `scratch`, a file-scope device array of
+  // kScratchRods bytes that every GPU the library runs on holds until the
+  // process ends (each GPU its own copy).  Its device address is looked up
+  // once per device and kept; the array's name is never used as an address
+  // in host code.  The flags the kernel needs cleared are cleared by a small
+  // kernel before each check (no memset node), and the result is copied
+  // straight into the caller's array.  Calls that overlap on one GPU share
+  // the array and race.
+
+  constexpr uint64_t kScratchRods = uint64_t(1) << 20;  // 1 MiB: one byte per rod
+  static __device__ __align__(256) uint8_t scratch[kScratchRods];
+
+  namespace {
+    constexpr int kMaxDevices = 64;
+    std::mutex g_scratch_mutex;
+    uint8_t* g_scratch_ptr[kMaxDevices] = {};
+
+    // the address of `scratch` on `device`, looked up on the first call there
+    int scratch_on(int device, uint8_t** out) {
+      if (device < 0 || device >= kMaxDevices) {
+        set_error("check_finite_noalloc: device ordinal out of range");
+        return EROD_ERR_INVALID;
+      }
+      std::lock_guard<std::mutex> lock(g_scratch_mutex);
+      if (!g_scratch_ptr[device]) {
+        int current = 0;
+        EROD_HIP(hipGetDevice(&current));
+        if (current != device) EROD_HIP(hipSetDevice(device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (current != device) EROD_HIP(hipSetDevice(current));
+        EROD_HIP(e);
+        g_scratch_ptr[device] = static_cast<uint8_t*>(p);
+      }
+      *out = g_scratch_ptr[device];
+      return EROD_OK;
+    }
+  }  // namespace
+
+  int fused_check_finite_noalloc(erod_batch* b, uint8_t* flags) {
+    if (b->n_rods > kScratchRods) {
+      set_error("erod_check_finite_noalloc: at most 1,048,576 rods (the size of its device scratch array)");
+      return EROD_ERR_INVALID;
+    }
+    uint8_t* d = nullptr;
+    if (int rc = scratch_on(b->device, &d)) return rc;
+    // rod records changed on the host go up whole (the staged upload of a
+    // few records would allocate its staging buffer)
+    if (b->rods_dirty || !b->dirty_rods.empty()) {
+      EROD_HIP(hipMemcpyAsync(b->d_rods, b->rods.data(), sizeof(RodInfo) * b->n_rods, hipMemcpyHostToDevice, b->stream));
+      b->rods_dirty = false;
+      b->dirty_rods.clear();
+    }
+    hipLaunchKernelGGL(k_zero_u8, dim3(grid_for(b->n_rods)), dim3(kThreads), 0, b->stream, d, b->n_rods);
+    const unsigned g = static_cast<unsigned>(b->n_rods < 65535 ? b->n_rods : 65535);
+    hipLaunchKernelGGL(k_check_finite<uint8_t>, dim3(g), dim3(256), 0, b->stream, b->d_field[EROD_POSITION],
+                       b->d_field[EROD_VELOCITY], b->d_field[EROD_DIRECTOR], b->d_field[EROD_ANGULAR_VELOCITY], b->d_rods,
+                       b->n_rods, b->E, d);
+    EROD_HIP(hipGetLastError());
+    EROD_HIP(hipMemcpyAsync(flags, d, b->n_rods, hipMemcpyDeviceToHost, b->stream));
+    EROD_HIP(hipStreamSynchronize(b->stream));
     return EROD_OK;
   }
 

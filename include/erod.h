@@ -290,6 +290,13 @@ int erod_load(const char* path, erod_batch** out);
 int erod_synchronize(erod_batch* b);
 /* Flags a non-finite value in x, v, Q or w of any rod (0/1 per rod). */
 int erod_check_finite(erod_batch* b, uint8_t* rod_flags);
+/* The same check without allocating device memory: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops).  The per-rod flags live in a 1 MiB device array the
+ * library holds on each GPU it runs on until the process ends, so a batch
+ * may have at most 1,048,576 rods (EROD_ERR_INVALID beyond).  Calls that
+ * overlap on one GPU share that array and race. */
+int erod_check_finite_noalloc(erod_batch* b, uint8_t* rod_flags);
 /* Message of the last failed call in this thread. */
 const char* erod_last_error(void);
 
