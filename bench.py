@@ -1,24 +1,27 @@
 """Benchmark: element-steps/s of the batched FP32-bulk Cosserat-rod step on the
-4096-rod ragged batch (n log-uniform in [16, 1024]) on one MI355X, beside the
-fused FP64 step, the plain (op-by-op) FP64 step and the CPU oracle.
+4096-rod ragged batch (n log-uniform in [16, 1024], each rod at its own
+stable dt) on one MI355X, beside the fused FP64 step, the plain (op-by-op)
+FP64 step and the CPU oracle.
 
-    python bench.py [--gpus 1] [--steps 10000] [--warmup 100]
+    python bench.py [--gpus 1] [--steps 10000] [--warmup 100] [--chunk 100]
 
-Prints ONE JSON line.  `value` is the FP32-bulk element-steps/s with the state
-resident in HBM (element-steps = sum of rod element counts x steps; ghost
-slots are not counted).  The roofline object prices the dominant kernel
-(the fused FP32-bulk step, `steps_per_launch` steps per launch) by its
-algorithmic bytes B32(n) = 360 n + 72 per rod-step times the steps of one
-launch, over the kernel's average launch duration, measured with HIP events
-on the batch stream around the timed region.  Because that kernel keeps the
-rod state on chip for all steps of a launch, its HBM traffic is far below the
-algorithmic bytes and the fraction can exceed 1; `roofline_streaming` prices
-the same kernel at one step per launch (state through HBM every step), and
-`traffic` holds the HBM bytes per launch measured with rocprofv3 counters
-(committed under profiles/, matched on the tile shape).  `roofline_valu`
-prices the same launch by what bounds it with the state on chip: its VALU
-instructions (the counter summary's instructions per wave and step) against
-one wave64 VALU instruction per CU per cycle.
+Prints ONE JSON line.  `value` is the FP32-bulk element-steps/s over the
+timed steps with the state resident in HBM (element-steps = sum of rod
+element counts x steps; ghost slots are not counted), replayed as hipGraph
+chunks of min(--chunk, --steps) steps per launch.
+
+`roofline` prices the dominant kernel (the fused FP32-bulk step) by what
+bounds it: it holds the rod state on chip for all steps of a launch, so HBM
+is touched once per launch and VALU issue bounds it.  `achieved` is its VALU
+instructions per launch (the committed rocprofv3 counter summary's
+instructions per wave and step x waves x steps per launch) over its average
+launch duration, measured with HIP events on the batch stream around the
+timed region; `peak` is one wave64 VALU instruction per CU per cycle.
+`traffic` and `hbm_measured` are its HBM bytes per launch from the counter
+summary (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on a plain device copy);
+`equivalent_streaming` is what a kernel streaming the state every step
+would move.  `roofline_streaming` prices the same kernel at one step per
+launch (state through HBM every step), an HBM-bound launch.
 """
 import argparse
 import json
@@ -104,17 +107,51 @@ def measured_traffic(lanes, epl, spl, tag="traffic", path=TRAFFIC_FILE, exact=Tr
 VALU_PEAK_GINSTR = 256 * 2.4
 
 
-def measured_valu(lanes, epl, spl, exact=True):
+def measured_valu(lanes, epl, spl, exact=True, tag="valu", path=TRAFFIC_FILE):
     """(waves per launch, VALU instructions per wave and step) of the
     FP32-bulk kernel at this tile shape on the ragged batch, from the
     rocprofv3 SQ counter summary (`valu lanes=L epl=P spl=S waves=W
-    insts_per_wave_step=I`), or None.  With exact=False the line of the
-    nearest steps per launch is taken and its step count returned third."""
-    c = _closest(_summary_lines(TRAFFIC_FILE, "valu", 6), lanes, epl, spl)
+    insts_per_wave_step=I`; `valu_contact ...` for the rods on a plane), or
+    None.  With exact=False the line of the nearest steps per launch is taken
+    and its step count returned third."""
+    c = _closest(_summary_lines(path, tag, 6), lanes, epl, spl)
     if c is None or (exact and c[0] != spl):
         return None
     r = (int(c[1]["waves"]), float(c[1]["insts_per_wave_step"]))
     return r if exact else r + (c[0],)
+
+
+def persistent_roofline(n, kms, launches, steps, lanes, epl, spl, traffic_tag="traffic", valu_tag="valu",
+                        path=TRAFFIC_FILE):
+    """The roofline of FP32-bulk launches that hold the state on chip for
+    their steps: HBM is touched once per launch (state in and out, constants
+    in), so what bounds a launch is VALU issue: its VALU instructions
+    (counter summary, per wave and step) over its event-timed duration,
+    against one wave64 VALU instruction per CU per cycle.  The measured HBM
+    traffic and the bytes a kernel streaming the state every step would move
+    are reported beside it."""
+    steps_launch = steps / launches
+    bytes_alg = float(b32(n).sum()) * steps_launch
+    traffic = measured_traffic(lanes, epl, spl, tag=traffic_tag, path=path, exact=False)
+    valu = measured_valu(lanes, epl, spl, exact=False, tag=valu_tag, path=path)
+    roofline = {"bound": "valu", "achieved": None, "peak": VALU_PEAK_GINSTR, "unit": "G wave64 VALU instr/s",
+                "frac": None, "traffic": traffic[0] if traffic else None,
+                "kernel": "k_fused<float,%d,%d> (FP32-bulk)" % (epl, lanes // epl),
+                "kernel_ms_per_launch": kms, "launches": launches, "steps_per_launch": steps_launch}
+    if valu is not None:
+        waves, ipws, valu_spl = valu
+        g = waves * ipws * steps_launch / (kms * 1e-3) / 1e9
+        roofline.update(achieved=g, frac=g / VALU_PEAK_GINSTR, insts_per_wave_step=ipws, waves_per_launch=waves,
+                        counters_at_steps_per_launch=valu_spl)
+    if traffic is not None:
+        hbm = traffic[0] / (kms * 1e-3) / 1e9
+        roofline["hbm_measured"] = {"achieved": hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm / HBM_PEAK_GBS,
+                                    "bytes_per_launch": traffic[0], "counters_at_steps_per_launch": traffic[1]}
+    roofline["equivalent_streaming"] = {
+        "bytes_per_launch": bytes_alg, "GB_per_s": bytes_alg / (kms * 1e-3) / 1e9, "unit": "GB/s",
+        "note": "B32(n) = 360 n + 72 bytes per rod-step times the launch's steps: what a kernel streaming the "
+                "state through HBM every step would move; not a fraction of peak (the state stays on chip)"}
+    return roofline
 
 
 def stream_copy_gbs(torch, nbytes=2 << 30, reps=10):
@@ -211,10 +248,11 @@ def contact_leg(args, torch):
     z = blk.get_field("Position")[2]
     zmin = float(min(z[s0:s0 + k + 1].min() for s0, k in zip(blk.starts, n)))  # rod nodes, not ghosts
     v = elems * args.contact_steps / el
-    bytes_launch = float(b32(n).sum()) * (args.contact_steps // launches)
-    achieved = bytes_launch / (kms * 1e-3) / 1e9
-    log("rods on a plane: %d rods x %d, %.3f s for %d steps -> %.3e element-steps/s, %.0f GB/s algorithmic"
-        % (len(n), n[0], el, args.contact_steps, v, achieved))
+    roofline = persistent_roofline(n, kms, launches, args.contact_steps, lanes, epl, spl, traffic_tag="traffic_contact",
+                                   valu_tag="valu_contact", path=CONTACT_TRAFFIC_FILE)
+    log("rods on a plane: %d rods x %d, %.3f s for %d steps -> %.3e element-steps/s, VALU issue %s"
+        % (len(n), n[0], el, args.contact_steps, v,
+           "%.2f" % roofline["frac"] if roofline["frac"] is not None else "(no counter summary for this shape)"))
     # plain FP64 steps continue from the fused state on the same batch
     blk.step(wl.dt, 2)
     sync()
@@ -229,11 +267,7 @@ def contact_leg(args, torch):
             "elements": elems, "dt": wl.dt, "steps": args.contact_steps, "value": v, "unit": "element-steps/s",
             "ms_per_step": el * 1e3 / args.contact_steps, "tile_lanes": lanes, "elements_per_thread": epl,
             "steps_per_launch": spl, "min_z": zmin,
-            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "kernel_ms_per_launch": kms, "launches": launches,
-                         "algorithmic_bytes_per_launch": bytes_launch,
-                         "traffic": measured_traffic(lanes, epl, spl, tag="traffic_contact",
-                                                     path=CONTACT_TRAFFIC_FILE)},
+            "roofline": roofline,
             "fp64_plain": {"value": vp, "steps": args.contact_plain_steps}, "ratio_fp32bulk_vs_plain_fp64": v / vp}
 
 
@@ -324,16 +358,51 @@ def small_batches_leg(args, torch, cpu):
     return out
 
 
+def _served_error_vs_fp64(wl, dts, sample):
+    """FP32-bulk served results against FP64: the sampled requests' rods
+    stepped again in one FP64 fused batch, each at its own dt for its
+    request's steps (step budgets), positions compared per rod relative to
+    its length.  `sample` = [(rods, steps, [Position of each rod])]."""
+    from elasticapp_amd import FP64
+    from elasticapp_amd.workloads import Workload, build_block
+    rods = [int(r) for rr, _, _ in sample for r in rr]
+    steps = [h for rr, h, _ in sample for _ in rr]
+    got = [x for _, _, xs in sample for x in xs]
+    force = {f[0]: f for f in wl.forces}
+    sub = Workload("served_sample", [wl.n_elems[r] for r in rods], [wl.states[r] for r in rods], wl.dt, 0,
+                   forces=[(i,) + tuple(force[r][1:]) for i, r in enumerate(rods) if r in force])
+    blk = build_block(sub)
+    blk.set_rod_dt(np.array([dts[r] for r in rods]))
+    for i, h in enumerate(steps):
+        blk.set_step_budget(i, h)
+    blk.step_fused_graph(max(steps), 100, FP64)
+    err = np.empty(len(rods))
+    for i, r in enumerate(rods):
+        L = float(wl.states[r]["ReferenceElementLength"].sum())
+        err[i] = np.abs(got[i] - blk.get_rod("Position", i)).max() / L
+    blk.close()
+    h = np.array(steps)
+    short = err[h <= 10000]
+    return {"max_over_rods": float(err.max()), "median_over_rods": float(np.median(err)), "rods": len(rods),
+            "requests": len(sample), "max_steps": int(max(steps)),
+            "max_over_rods_horizon_le_1e4": float(short.max()) if short.size else None,
+            "rods_horizon_le_1e4": int(short.size),
+            "note": "positions of every rod of every 40th served request (FP32-bulk, 500 steps per tick) against "
+                    "the same rods stepped by the fused FP64 path, |dx| / rod length"}
+
+
 def serving_leg(args, torch, capacity):
     """Continuous batching: Poisson arrivals of requests (1-64 template rods of
-    32-512 elements, horizons of 1e3-1e5 steps) served from a pool of rod slots
-    by the FP32-bulk fused kernel; latency = completion - arrival.  The arrival
-    rate is fixed (`--serve-rate`, requests per second), so that p50 / p99 are
-    comparable between runs and builds; the server's own capacity, measured
-    first as the element-steps per second it advances with a backlog that
-    keeps its queue non-empty, is reported beside it with the load the rate
-    offers (`capacity` is the batched rate of the ragged batch, for
-    comparison)."""
+    32-512 elements, horizons of 1e3-1e5 steps) served by the FP32-bulk fused
+    kernel from a pool of rod slots that grows on demand within the 200 GB
+    HBM budget (erod_server_create_budgeted; requests admitted in order, a
+    waiting request reserving its slot classes); latency = completion -
+    arrival.  The arrival rate is fixed (`--serve-rate`, requests per
+    second), so that p50 / p99 are comparable between runs and builds; the
+    server's own capacity, measured first as the element-steps per second it
+    advances on a backlog of requests, is reported beside it with the load the
+    rate offers (`capacity` is the batched rate of the ragged batch, for
+    comparison).  Every 40th request's result is checked against FP64."""
     from elasticapp_amd import FP32_BULK, Server
     from elasticapp_amd.workloads import build_block, serving_library, serving_requests
     wl, dts = serving_library()
@@ -344,32 +413,37 @@ def serving_leg(args, torch, capacity):
     work = np.array([n_lib[r].sum() * h for r, h in reqs], dtype=np.float64)
     # mean work of the request distribution (sampled over many draws, not just this run's)
     mean_work = float(np.mean([n_lib[r].sum() * h for r, h in serving_requests(20000, len(n_lib), seed=99)]))
-    classes = [(64, 8192), (128, 8192), (256, 8192), (512, 8192)]
-    # the server's capacity: a backlog keeps the queue non-empty over the timed ticks
+    capacities = [64, 128, 256, 512]
+    budget = 200e9
     spt = args.serve_steps_per_tick
-    sat = Server(lib, classes, precision=FP32_BULK, steps_per_tick=spt, hbm_budget_bytes=200e9)
+    # the server's capacity: a backlog of 3,000 requests, admitted as the
+    # budget allows; 20 ticks timed after 3 that admit and grow the pool
+    sat = Server.budgeted(lib, capacities, precision=FP32_BULK, steps_per_tick=spt, hbm_budget_bytes=budget)
     for i, (rods, h) in enumerate(serving_requests(3000, len(n_lib), seed=3006)):
         sat.submit(i, rods, h)
-    for _ in range(20):
+    for _ in range(3):
         for rid in sat.tick():
             sat.release(rid)
     t0 = time.perf_counter()
     k0, e0 = sat.stats()
-    while sat.status()["queued"] > 0 and sat.stats()[0] - k0 < 200:
+    for _ in range(20):
         for rid in sat.tick():
             sat.release(rid)
     sat_wall = time.perf_counter() - t0
     k1, e1 = sat.stats()
     server_capacity = (e1 - e0) / sat_wall
+    sat_active = sat.status()["active_elements"]
     sat.close()
-    log("serving, server capacity: %.3e element-steps/s over %d backlogged ticks" % (server_capacity, k1 - k0))
+    log("serving, server capacity: %.3e element-steps/s over %d ticks with %d active elements"
+        % (server_capacity, k1 - k0, sat_active))
     lam = args.serve_rate
     rng = np.random.default_rng(3005)
     arrivals = np.cumsum(rng.exponential(1.0 / lam, len(reqs)))
-    srv = Server(lib, classes, precision=FP32_BULK, steps_per_tick=spt, hbm_budget_bytes=200e9)
-    pool_bytes = srv.status()["pool_bytes"]
+    srv = Server.budgeted(lib, capacities, precision=FP32_BULK, steps_per_tick=spt, hbm_budget_bytes=budget)
     lat = np.full(len(reqs), np.nan)
     nxt = done = ticks = non_finite = 0
+    max_active = max_pool = 0
+    sample = []
     t0 = time.perf_counter()
     while done < len(reqs):
         now = time.perf_counter() - t0
@@ -381,29 +455,48 @@ def serving_leg(args, torch, capacity):
             time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
             continue
         for rid in srv.tick():
+            lat[rid] = time.perf_counter() - t0 - arrivals[rid]
             x = srv.read(rid, 0, "Position")     # the result leaves the device
             non_finite += int(not np.isfinite(x).all())
+            if rid % 40 == 0:
+                rods, h = reqs[rid]
+                sample.append((rods, h, [srv.read(rid, k, "Position") for k in range(len(rods))]))
             srv.release(rid)
-            lat[rid] = time.perf_counter() - t0 - arrivals[rid]
             done += 1
+        st = srv.status()
+        max_active = max(max_active, st["active_elements"])
+        max_pool = max(max_pool, st["pool_bytes"])
         ticks += 1
     wall = time.perf_counter() - t0
+    info = srv.pool_info()
     srv.close()
+    err = _served_error_vs_fp64(wl, dts, sample) if sample else None
     lib.close()
     thr = float(work.sum()) / wall
     out = {"workload": "%d requests, U{1..64} rods of %d templates (n logU[32,512], materials as the ragged batch, "
                        "own stable dt), horizons logU[1e3,1e5] steps, Poisson arrivals" % (len(reqs), len(n_lib)),
            "lambda_per_s": lam, "lambda": "fixed", "server_capacity_element_steps_per_s": server_capacity,
-           "server_capacity_ticks": k1 - k0, "ragged_batch_element_steps_per_s": capacity,
+           "server_capacity_ticks": k1 - k0, "server_capacity_active_elements": sat_active,
+           "ragged_batch_element_steps_per_s": capacity,
            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
            "mean_s": float(np.mean(lat)), "max_s": float(np.max(lat)), "throughput": thr, "unit": "element-steps/s",
            "non_finite_results": non_finite,
            "utilization": thr / server_capacity, "offered_load_fraction": lam * mean_work / server_capacity,
-           "wall_s": wall,
-           "ticks": ticks, "steps_per_tick": spt,
-           "pool_classes": classes, "pool_bytes": pool_bytes, "hbm_budget_bytes": 200e9}
+           "wall_s": wall, "ticks": ticks, "steps_per_tick": spt,
+           "admission": "HBM budget (pool grows on demand), in order, a waiting request reserving its classes",
+           "slot_classes": capacities, "pool_growths": info["growths"], "pool_slots": info["slots"],
+           "pool_bytes_max": max_pool, "active_elements_max": max_active, "hbm_budget_bytes": budget,
+           "fp32_bulk_position_err_vs_fp64": err}
     log("serving: lambda %.1f/s, p50 %.3f s, p99 %.3f s, %.3e element-steps/s (%.0f%% of the server's capacity), "
-        "%d ticks" % (lam, out["p50_s"], out["p99_s"], thr, 100 * thr / server_capacity, ticks))
+        "%d ticks, pool grown %d times to %d slots (%.2f GB)"
+        % (lam, out["p50_s"], out["p99_s"], thr, 100 * thr / server_capacity, ticks, info["growths"], info["slots"],
+           max_pool / 1e9))
+    if err:
+        log("serving: fp32-bulk vs fp64 on %d rods of %d requests: max |dx|/L %.3e, median %.3e; "
+            "%d rods with horizons up to 1e4 steps: max %s"
+            % (err["rods"], err["requests"], err["max_over_rods"], err["median_over_rods"],
+               err["rods_horizon_le_1e4"], "%.3e" % err["max_over_rods_horizon_le_1e4"]
+               if err["max_over_rods_horizon_le_1e4"] is not None else "-"))
     return out
 
 
@@ -560,36 +653,26 @@ def main():
     achieved_s = bytes_s / (kms_s * 1e-3) / 1e9
     log("fp32-bulk at 1 step/launch: %.4f ms/launch -> %.0f GB/s algorithmic (%.3e element-steps/s)"
         % (kms_s, achieved_s, elems / (kms_s * 1e-3)))
+    # the cost of the scheduled director re-orthonormalisation (one
+    # Newton-Schulz step over every rod's directors: 72 B read and written
+    # per element), against the sustained step time at one every 1,000 steps
+    sync32()
+    reps = 20
+    t = time.perf_counter()
+    for _ in range(reps):
+        blk32.reorthonormalize()
+    sync32()
+    ro_ms = (time.perf_counter() - t) * 1e3 / reps
+    step_ms = (kms_s32 / args.chunk) if sustained else (kms / (args.steps / launches))
+    reortho = {"ms_per_call": ro_ms, "every": 1000, "cost_fraction": ro_ms / (1000 * step_ms),
+               "bytes_per_call": 144.0 * elems, "GB_per_s": 144.0 * elems / (ro_ms * 1e-3) / 1e9,
+               "note": "host-timed over %d calls; scheduled on the fused paths by erod_set_reortho_interval" % reps}
+    log("reorthonormalisation: %.4f ms per call, %.2f%% of the step time at one every 1,000 steps"
+        % (ro_ms, 100 * reortho["cost_fraction"]))
     blk32.close()
 
     # ---- roofline of the timed launches ----------------------------------------------------
-    # With the state held on chip for the steps of a launch, HBM is touched
-    # once per launch (state in and out, constants in), so what bounds the
-    # launch is VALU issue: its VALU instructions (counter summary, per wave
-    # and step) over its event-timed duration, against one wave64 VALU
-    # instruction per CU per cycle.  The measured HBM traffic and the bytes a
-    # streaming kernel would move for the same steps are reported beside it.
-    bytes_alg = float(b32(n).sum()) * (args.steps / launches)
-    traffic = measured_traffic(lanes, epl, spl, exact=False)
-    valu = measured_valu(lanes, epl, spl, exact=False)
-    steps_launch = args.steps / launches
-    roofline = {"bound": "valu", "achieved": None, "peak": VALU_PEAK_GINSTR, "unit": "G wave64 VALU instr/s",
-                "frac": None, "traffic": traffic[0] if traffic else None,
-                "kernel": "k_fused<float,%d,%d> (FP32-bulk)" % (epl, lanes // epl),
-                "kernel_ms_per_launch": kms, "launches": launches, "steps_per_launch": steps_launch}
-    if valu is not None:
-        waves, ipws, valu_spl = valu
-        g = waves * ipws * steps_launch / (kms * 1e-3) / 1e9
-        roofline.update(achieved=g, frac=g / VALU_PEAK_GINSTR, insts_per_wave_step=ipws, waves_per_launch=waves,
-                        counters_at_steps_per_launch=valu_spl)
-    if traffic is not None:
-        hbm = traffic[0] / (kms * 1e-3) / 1e9
-        roofline["hbm_measured"] = {"achieved": hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm / HBM_PEAK_GBS,
-                                    "bytes_per_launch": traffic[0], "counters_at_steps_per_launch": traffic[1]}
-    roofline["equivalent_streaming"] = {
-        "bytes_per_launch": bytes_alg, "GB_per_s": bytes_alg / (kms * 1e-3) / 1e9, "unit": "GB/s",
-        "note": "B32(n) = 360 n + 72 bytes per rod-step times the launch's steps: what a kernel streaming the "
-                "state through HBM every step would move; not a fraction of peak (the state stays on chip)"}
+    roofline = persistent_roofline(n, kms, launches, args.steps, lanes, epl, spl)
 
     # ---- plain FP64 (one kernel per reference operator) ----------------------------------
     blkp = make_block(wl)
@@ -632,6 +715,7 @@ def main():
                    "tile_lanes": lanes, "elements_per_thread": epl, "second_group": list(split32),
                    "device": dev_name},
         "roofline": roofline,
+        "reortho": reortho,
         "roofline_streaming": {"bound": "hbm", "achieved": achieved_s, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": achieved_s / HBM_PEAK_GBS,
                                "traffic": measured_traffic(lanes, epl, 1),

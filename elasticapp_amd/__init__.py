@@ -103,6 +103,8 @@ def load_library():
         "erod_step_fused_graph": (i32, [vp, i32, u64, u64]),
         "erod_capture_fused_graph": (i32, [vp, i32, u64]),
         "erod_reorthonormalize": (i32, [vp]),
+        "erod_set_reortho_interval": (i32, [vp, u64]),
+        "erod_get_reortho_interval": (i32, [vp, P(u64), P(u64)]),
         "erod_last_kernel_ms": (i32, [vp, P(dbl), P(u64)]),
         "erod_synchronize": (i32, [vp]),
         "erod_check_finite": (i32, [vp, P(ctypes.c_uint8)]),
@@ -115,6 +117,9 @@ def load_library():
         "erod_get_rod_step_budget": (i32, [vp, u64, P(u64)]),
         "erod_copy_rods": (i32, [vp, P(u64), vp, P(u64), u64]),
         "erod_server_create": (i32, [vp, P(u64), P(u64), i32, i32, i32, dbl, P(vp)]),
+        "erod_server_create_budgeted": (i32, [vp, P(u64), i32, i32, i32, dbl, P(vp)]),
+        "erod_server_pool_info": (i32, [vp, P(u64), P(u64), P(dbl)]),
+        "erod_batch_rods": (i32, [vp, P(u64), P(u64)]),
         "erod_server_destroy": (i32, [vp]),
         "erod_server_pool": (i32, [vp, P(vp)]),
         "erod_server_submit": (i32, [vp, u64, P(u64), u64, u64]),
@@ -366,6 +371,19 @@ class DeviceBlock:
     def reorthonormalize(self):
         _check(_LIB.erod_reorthonormalize(self._h))
 
+    def set_reortho_interval(self, every):
+        """Re-orthonormalise the directors of the rods with steps left every
+        `every` fused steps (erod_set_reortho_interval; 0: never)."""
+        if int(every) < 0:
+            raise ValueError("the interval is a number of steps, not %r" % (every,))
+        _check(_LIB.erod_set_reortho_interval(self._h, int(every)))
+
+    def reortho_interval(self):
+        """(interval, steps since the last scheduled re-orthonormalisation)"""
+        e, p = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(_LIB.erod_get_reortho_interval(self._h, ctypes.byref(e), ctypes.byref(p)))
+        return e.value, p.value
+
     def last_kernel_ms(self):
         ms, n = ctypes.c_double(), ctypes.c_uint64()
         _check(_LIB.erod_last_kernel_ms(self._h, ctypes.byref(ms), ctypes.byref(n)))
@@ -521,34 +539,65 @@ class _PoolView(DeviceBlock):
 class Server:
     """Continuous batching of requests over a pool of rod slots (erod_server_*).
 
-    `classes` is a list of (slot capacity in elements, slot count); a request
-    is a list of rods of `library` (a DeviceBlock holding the template rods,
-    with their dt set) and a number of steps."""
+    `classes` is a list of (slot capacity in elements, slot count): a fixed
+    pool (erod_server_create).  `Server.budgeted(library, capacities, ...)`
+    makes a server whose pool grows within its HBM budget instead
+    (erod_server_create_budgeted).  A request is a list of rods of `library`
+    (a DeviceBlock holding the template rods, with their dt set) and a number
+    of steps."""
 
-    def __init__(self, library, classes, precision=1, steps_per_tick=100, hbm_budget_bytes=200e9):
+    def __init__(self, library, classes, precision=1, steps_per_tick=100, hbm_budget_bytes=200e9, _budgeted=False):
         L = load_library()
-        caps = np.array([c for c, _ in classes], dtype=np.uint64)
-        cnts = np.array([k for _, k in classes], dtype=np.uint64)
         h = ctypes.c_void_p()
-        _check(L.erod_server_create(library._h, _u64p(caps), _u64p(cnts), len(classes), precision, steps_per_tick,
-                                    float(hbm_budget_bytes), ctypes.byref(h)))
+        if _budgeted:
+            caps = np.array([int(c) for c in classes], dtype=np.uint64)
+            _check(L.erod_server_create_budgeted(library._h, _u64p(caps), len(caps), precision, steps_per_tick,
+                                                 float(hbm_budget_bytes), ctypes.byref(h)))
+        else:
+            caps = np.array([c for c, _ in classes], dtype=np.uint64)
+            cnts = np.array([k for _, k in classes], dtype=np.uint64)
+            _check(L.erod_server_create(library._h, _u64p(caps), _u64p(cnts), len(classes), precision, steps_per_tick,
+                                        float(hbm_budget_bytes), ctypes.byref(h)))
         self._h = h
         self.library = library
-        p = ctypes.c_void_p()
-        _check(L.erod_server_pool(self._h, ctypes.byref(p)))
-        n_elems = []
-        for cap, k in sorted(classes):
-            n_elems += [int(cap)] * int(k)
-        self.pool = _PoolView(p, n_elems, library.damped)
+        self._pool = None
         self._done = np.zeros(4096, dtype=np.uint64)
         self._lib_n = [library.active_length(r) for r in range(len(library.n_elems))]
         self._sizes = {}
 
+    @classmethod
+    def budgeted(cls, library, capacities, precision=1, steps_per_tick=100, hbm_budget_bytes=200e9):
+        """A server whose pool of slots (classes of the given capacities)
+        grows on demand while it fits `hbm_budget_bytes`."""
+        return cls(library, capacities, precision, steps_per_tick, hbm_budget_bytes, _budgeted=True)
+
+    @property
+    def pool(self):
+        """A view of the current pool batch (a new one after the pool grew)."""
+        p = ctypes.c_void_p()
+        _check(_LIB.erod_server_pool(self._h, ctypes.byref(p)))
+        if self._pool is None or self._pool._h is None or self._pool._h.value != p.value:
+            if self._pool is not None:
+                self._pool.close()
+            n = ctypes.c_uint64()
+            _check(_LIB.erod_batch_rods(p, ctypes.byref(n), None))
+            caps = np.zeros(n.value, dtype=np.uint64)
+            _check(_LIB.erod_batch_rods(p, ctypes.byref(n), _u64p(caps)))
+            self._pool = _PoolView(p, [int(c) for c in caps], self.library.damped)
+        return self._pool
+
+    def pool_info(self):
+        """{"slots", "growths", "hbm_budget_bytes"} of the pool."""
+        n, g, b = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_double()
+        _check(_LIB.erod_server_pool_info(self._h, ctypes.byref(n), ctypes.byref(g), ctypes.byref(b)))
+        return {"slots": n.value, "growths": g.value, "hbm_budget_bytes": b.value}
+
     def close(self):
         if getattr(self, "_h", None):
             # the pool view goes first: its handle dies with the server
-            if getattr(self, "pool", None) is not None:
-                self.pool.close()
+            if getattr(self, "_pool", None) is not None:
+                self._pool.close()
+                self._pool = None
             _LIB.erod_server_destroy(self._h)
             self._h = None
 

@@ -10,6 +10,7 @@ import numpy as np
 
 MAGIC, END = b"ERODCKP1", b"ERODEND1"
 _ROD = struct.Struct("<3dI3d9d3di2d")
+_CLAMP2 = struct.Struct("<2I3d9d")   # version 3: clamp indices, the second clamp's pose
 
 
 def _field_names():
@@ -23,7 +24,7 @@ def read_header(path):
         if f.read(8) != MAGIC:
             raise ValueError("not an erod checkpoint: %s" % path)
         version, damped, n_rods, _ = struct.unpack("<IIQQ", f.read(24))
-        if version not in (1, 2):
+        if version not in (1, 2, 3):
             raise ValueError("unsupported checkpoint version %d" % version)
         raw = f.read(8 * n_rods)
         if len(raw) != 8 * n_rods:
@@ -41,7 +42,7 @@ def read(path):
     o = 8
     version, damped, n_rods, slots = struct.unpack_from("<IIQQ", data, o)
     o += 24
-    if version not in (1, 2):
+    if version not in (1, 2, 3):
         raise ValueError("unsupported checkpoint version %d" % version)
 
     def arr(dtype, n):
@@ -61,9 +62,16 @@ def read(path):
     for _ in range(n_rods):
         v = _ROD.unpack_from(data, o)
         o += _ROD.size
-        rods.append({"dt": v[0], "gamma_f": v[1], "gamma_t": v[2], "flags": v[3], "clamp_x": np.array(v[4:7]),
-                     "clamp_q": np.array(v[7:16]), "force": np.array(v[16:19]), "ramp": v[19],
-                     "ramp_start": v[20], "ramp_interval": v[21]})
+        rod = {"dt": v[0], "gamma_f": v[1], "gamma_t": v[2], "flags": v[3], "clamp_x": np.array(v[4:7]),
+               "clamp_q": np.array(v[7:16]), "force": np.array(v[16:19]), "ramp": v[19],
+               "ramp_start": v[20], "ramp_interval": v[21], "clamp_index": (0, 0)}
+        # version 3: the clamps' indices and a second clamp (flag bit 3)
+        if version >= 3:
+            w = _CLAMP2.unpack_from(data, o)
+            o += _CLAMP2.size
+            rod["clamp_index"] = (w[0], w[1])
+            rod["clamp1_x"], rod["clamp1_q"] = np.array(w[2:5]), np.array(w[5:14])
+        rods.append(rod)
     out["rods"] = rods
     grav, = struct.unpack_from("<i", data, o)
     o += 4

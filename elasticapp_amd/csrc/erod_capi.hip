@@ -267,6 +267,14 @@ int erod_batch_slots(const erod_batch* b, uint64_t* element_slots) {
   return EROD_OK;
 }
 
+int erod_batch_rods(const erod_batch* b, uint64_t* n_rods, uint64_t* capacity) {
+  if (check_batch(b) || !n_rods) return EROD_ERR_INVALID;
+  *n_rods = b->n_rods;
+  if (capacity)
+    for (uint64_t r = 0; r < b->n_rods; ++r) capacity[r] = b->n_elems[r];
+  return EROD_OK;
+}
+
 int erod_rod_start(const erod_batch* b, uint64_t rod, uint64_t* start) {
   if (check_batch(b) || !start) return EROD_ERR_INVALID;
   if (rod >= b->n_rods) {
@@ -370,14 +378,32 @@ int erod_add_clamp(erod_batch* b, uint64_t rod, uint64_t index, const double x[3
     set_error("erod_add_clamp: bad argument");
     return EROD_ERR_INVALID;
   }
-  if (index != 0) {
-    set_error("erod_add_clamp: only index 0 (the rod's first node / element) is supported");
-    return EROD_ERR_UNSUPPORTED;
+  if (index > b->n_active[rod]) {
+    set_error("erod_add_clamp: index past the rod's last node");
+    return EROD_ERR_INVALID;
   }
   RodInfo& ri = b->rods[rod];
-  ri.flags |= 1u;
-  std::memcpy(ri.clamp_x, x, sizeof(ri.clamp_x));
-  std::memcpy(ri.clamp_q, Q, sizeof(ri.clamp_q));
+  const uint64_t n = b->n_active[rod];
+  // clamps at n - 1 and n would both hold element n - 1
+  for (int c = 0; c < 2; ++c)
+    if ((ri.flags & clamp_bit(c)) && ((index == n && ri.clamp_idx[c] + 1 == n) || (index + 1 == n && ri.clamp_idx[c] == n))) {
+      set_error("erod_add_clamp: clamps at n - 1 and n would both hold the rod's last element");
+      return EROD_ERR_INVALID;
+    }
+  // a clamp at an index that has one replaces it; otherwise it takes a free one of the two
+  int k = -1;
+  for (int c = 0; c < 2 && k < 0; ++c)
+    if ((ri.flags & clamp_bit(c)) && ri.clamp_idx[c] == index) k = c;
+  for (int c = 0; c < 2 && k < 0; ++c)
+    if (!(ri.flags & clamp_bit(c))) k = c;
+  if (k < 0) {
+    set_error("erod_add_clamp: a rod takes at most two clamps");
+    return EROD_ERR_UNSUPPORTED;
+  }
+  ri.flags |= clamp_bit(k);
+  ri.clamp_idx[k] = static_cast<uint32_t>(index);
+  std::memcpy(ri.clamp[k].x, x, sizeof(ri.clamp[k].x));
+  std::memcpy(ri.clamp[k].q, Q, sizeof(ri.clamp[k].q));
   b->has_clamps = true;
   mark_rod(b, rod);
   b->graph_precision = -1;
@@ -631,7 +657,7 @@ int erod_copy_rods(erod_batch* dst, const uint64_t* dst_rods, erod_batch* src, c
     dst->rods[p.dst_rod] = ri;
     dst->n_active[p.dst_rod] = p.n;
     dst->steps_left[p.dst_rod] = kNoBudget;
-    dst->has_clamps |= (ri.flags & 1u) != 0;
+    dst->has_clamps |= (ri.flags & kClampBits) != 0;
     dst->has_forces |= (ri.flags & 2u) != 0;
     mark_rod(dst, p.dst_rod);
   }
@@ -718,6 +744,16 @@ int erod_autotune_fused(erod_batch* b, int precision, int trial_steps, double* b
   return fused_autotune(b, precision, trial_steps, best_ms_per_step);
 }
 
+// the scheduled re-orthonormalisation after a launch of s steps: when the
+// batch's step count reaches a multiple of the interval during the launch
+static int after_launch(erod_batch* b, uint64_t s) {
+  if (b->reortho_every == 0) return EROD_OK;
+  b->reortho_phase += s;
+  if (b->reortho_phase < b->reortho_every) return EROD_OK;
+  b->reortho_phase %= b->reortho_every;
+  return fused_reorthonormalize_live(b);
+}
+
 int erod_step_fused(erod_batch* b, int precision, uint64_t n) {
   if (check_batch(b)) return EROD_ERR_INVALID;
   int rc = fused_common(b, precision);
@@ -730,6 +766,7 @@ int erod_step_fused(erod_batch* b, int precision, uint64_t n) {
     const uint64_t s = std::min<uint64_t>(spl, n - done);
     if ((rc = fused_launch(b, precision, static_cast<int>(s), false))) return rc;
     spend_budgets(b, s);
+    if ((rc = after_launch(b, s))) return rc;
     ++b->last_launches;
     done += s;
   }
@@ -737,24 +774,36 @@ int erod_step_fused(erod_batch* b, int precision, uint64_t n) {
   return EROD_OK;
 }
 
-// capture (or keep) the graph of one chunk of `chunk` fused steps
+// capture (or keep) the graph of one chunk of `chunk` fused steps, with the
+// scheduled re-orthonormalisations that fall into it from the current phase
+// (a chunk that is a multiple of the interval starts every replay at the
+// same phase; otherwise a replay at another phase captures again)
 static int ensure_graph(erod_batch* b, int precision, uint64_t chunk) {
   const uint64_t spl = b->plan.steps_per_launch > 0 ? uint64_t(b->plan.steps_per_launch) : 100;
-  if (b->graph_precision != precision || b->graph_chunk != chunk || !b->graph_exec) {
+  const bool same_schedule =
+      b->graph_reortho[0] == b->reortho_every && (b->reortho_every == 0 || b->graph_reortho[1] == b->reortho_phase);
+  if (b->graph_precision != precision || b->graph_chunk != chunk || !b->graph_exec || !same_schedule) {
     if (b->graph_exec) {
+      // (an earlier replay may still run on the stream)
+      EROD_HIP(hipStreamSynchronize(b->stream));
       hipGraphExecDestroy(b->graph_exec);
       b->graph_exec = nullptr;
     }
     hipGraph_t g = nullptr;
+    const uint64_t phase0 = b->reortho_phase;
     EROD_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     uint64_t done = 0, launches = 0;
     int lrc = EROD_OK;
     while (done < chunk && lrc == EROD_OK) {
       const uint64_t s = std::min<uint64_t>(spl, chunk - done);
       lrc = fused_launch(b, precision, static_cast<int>(s), false);
+      if (lrc == EROD_OK) lrc = after_launch(b, s);
       done += s;
       ++launches;
     }
+    // the capture ran nothing: the phase moves when the graph is replayed
+    const uint64_t phase1 = b->reortho_phase;
+    b->reortho_phase = phase0;
     hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (lrc || e != hipSuccess) {
       if (g) hipGraphDestroy(g);
@@ -769,6 +818,9 @@ static int ensure_graph(erod_batch* b, int precision, uint64_t chunk) {
     b->graph_precision = precision;
     b->graph_chunk = chunk;
     b->graph_launches = launches;
+    b->graph_reortho[0] = b->reortho_every;
+    b->graph_reortho[1] = phase0;
+    b->graph_phase_after = phase1;
   }
   return EROD_OK;
 }
@@ -794,8 +846,10 @@ int erod_step_fused_graph(erod_batch* b, int precision, uint64_t n, uint64_t chu
   if (whole > 0 && (rc = ensure_graph(b, precision, chunk))) return rc;
   EROD_HIP(hipEventRecord(b->ev0, b->stream));
   for (uint64_t i = 0; i < whole; ++i) {
+    if (i > 0 && b->reortho_every && (rc = ensure_graph(b, precision, chunk))) return rc;
     EROD_HIP(hipGraphLaunch(b->graph_exec, b->stream));
     for (uint64_t d = 0; d < chunk; d += spl) spend_budgets(b, std::min<uint64_t>(spl, chunk - d));
+    b->reortho_phase = b->graph_phase_after;
   }
   uint64_t launches = whole * b->graph_launches;
   // the steps that do not fill a chunk: direct launches of the same length
@@ -806,6 +860,7 @@ int erod_step_fused_graph(erod_batch* b, int precision, uint64_t n, uint64_t chu
     const uint64_t s = std::min<uint64_t>(spl, rest - done);
     if ((rc = fused_launch(b, precision, static_cast<int>(s), false))) return rc;
     spend_budgets(b, s);
+    if ((rc = after_launch(b, s))) return rc;
     ++launches;
     done += s;
   }
@@ -817,6 +872,20 @@ int erod_step_fused_graph(erod_batch* b, int precision, uint64_t n, uint64_t chu
 int erod_reorthonormalize(erod_batch* b) {
   if (check_batch(b)) return EROD_ERR_INVALID;
   return fused_reorthonormalize(b);
+}
+
+int erod_set_reortho_interval(erod_batch* b, uint64_t every) {
+  if (check_batch(b)) return EROD_ERR_INVALID;
+  b->reortho_every = every;
+  b->reortho_phase = 0;
+  return EROD_OK;
+}
+
+int erod_get_reortho_interval(const erod_batch* b, uint64_t* every, uint64_t* phase) {
+  if (check_batch(b)) return EROD_ERR_INVALID;
+  if (every) *every = b->reortho_every;
+  if (phase) *phase = b->reortho_phase;
+  return EROD_OK;
 }
 
 int erod_last_kernel_ms(const erod_batch* b, double* ms, uint64_t* launches) {

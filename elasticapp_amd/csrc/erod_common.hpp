@@ -55,16 +55,24 @@ namespace erod {
   struct RodInfo {
     uint64_t start;   // slot of node 0 / element 0 / voronoi 0
     uint32_t n;       // elements
-    uint32_t flags;   // bit0 clamp at index 0, bit1 end force, bit2 damping rates that vary along the rod
+    uint32_t flags;   // bit0 clamp 0, bit1 end force, bit2 damping rates that vary along the rod, bit3 clamp 1
     double dt;        // timestep of this rod (fused path)
     double gamma_f;   // force damping rate (when the same along the rod)
     double gamma_t;   // torque damping rate (when the same along the rod)
-    double clamp_x[3];
-    double clamp_q[9];
+    // up to two clamps: node clamp_idx[k] held at clamp[k].x, and element
+    // clamp_idx[k] (when below n) at director clamp[k].q
+    struct Pose {
+      double x[3];
+      double q[9];
+    } clamp[2];
+    uint32_t clamp_idx[2];
     double force[3];
     int ramp;
     double ramp_start, ramp_interval;
   };
+  // flag bit of clamp k
+  constexpr uint32_t clamp_bit(int k) { return k == 0 ? 1u : 8u; }
+  constexpr uint32_t kClampBits = 1u | 8u;
 
   struct Env {
     int gravity;
@@ -173,6 +181,12 @@ struct erod_batch {
   uint64_t graph_launches = 0;  // kernel launches in one replay of graph_exec
   int graph_precision = -1;
   uint64_t graph_chunk = 0;
+  uint64_t graph_reortho[2] = {0, 0};  // interval and phase the graph was captured at
+  uint64_t graph_phase_after = 0;      // phase after one replay
+  // scheduled re-orthonormalisation of the directors on the fused paths:
+  // after the launch during which the batch's step count reaches a multiple
+  // of reortho_every (0: never); reortho_phase = steps since the last one
+  uint64_t reortho_every = 0, reortho_phase = 0;
   double* d_time_next = nullptr;  // rod clocks written by launches over pieces, copied back after each
   // second copy of x, v, Q, w that launches over pieces alternate with (same layout as the fields)
   double* d_shadow[4] = {};
@@ -231,6 +245,9 @@ namespace erod {
   int fused_prepare(erod_batch* b, int precision);
   int fused_launch(erod_batch* b, int precision, int steps, bool timed);
   int fused_reorthonormalize(erod_batch* b);
+  // the directors of every rod with steps left to take (no budget, or one
+  // not spent), after a launch on the batch's stream
+  int fused_reorthonormalize_live(erod_batch* b);
   int fused_check_finite(erod_batch* b, uint8_t* flags);
   int fused_check_finite_noalloc(erod_batch* b, uint8_t* flags);
   int fused_autotune(erod_batch* b, int precision, int trial_steps, double* best_ms);

@@ -40,20 +40,35 @@ namespace erod {
         o[i] = static_cast<float>(a[i]);
     }
 
-    // one Newton-Schulz polar step Q <- Q (3 I - Q^T Q) / 2 on every element slot
+    // one Newton-Schulz polar step Q <- Q (3 I - Q^T Q) / 2 on element slot k
+    __device__ __forceinline__ void reortho_slot(double* __restrict__ Q, uint64_t E, uint64_t k) {
+      double q[9], g[9], o[9];
+      for (int m = 0; m < 9; ++m) q[m] = Q[m * E + k];
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) g[3 * a + b] = q[a] * q[b] + q[3 + a] * q[3 + b] + q[6 + a] * q[6 + b];
+      for (int p = 0; p < 3; ++p)
+        for (int r = 0; r < 3; ++r) {
+          double s = 0.0;
+          for (int c = 0; c < 3; ++c) s += q[3 * p + c] * ((c == r ? 3.0 : 0.0) - g[3 * c + r]);
+          o[3 * p + r] = 0.5 * s;
+        }
+      for (int m = 0; m < 9; ++m) Q[m * E + k] = o[m];
+    }
+
+    // every element slot
     __global__ void k_reortho(double* __restrict__ Q, uint64_t E) {
-      for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < E; k += (uint64_t)gridDim.x * blockDim.x) {
-        double q[9], g[9], o[9];
-        for (int m = 0; m < 9; ++m) q[m] = Q[m * E + k];
-        for (int a = 0; a < 3; ++a)
-          for (int b = 0; b < 3; ++b) g[3 * a + b] = q[a] * q[b] + q[3 + a] * q[3 + b] + q[6 + a] * q[6 + b];
-        for (int p = 0; p < 3; ++p)
-          for (int r = 0; r < 3; ++r) {
-            double s = 0.0;
-            for (int c = 0; c < 3; ++c) s += q[3 * p + c] * ((c == r ? 3.0 : 0.0) - g[3 * c + r]);
-            o[3 * p + r] = 0.5 * s;
-          }
-        for (int m = 0; m < 9; ++m) Q[m * E + k] = o[m];
+      for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < E; k += (uint64_t)gridDim.x * blockDim.x)
+        reortho_slot(Q, E, k);
+    }
+
+    // the active elements of every rod with steps left (steps_left null: all
+    // rods); a rod whose budget is spent keeps its state bit for bit
+    __global__ void k_reortho_live(double* __restrict__ Q, uint64_t E, const RodInfo* __restrict__ rods, uint64_t n_rods,
+                                   const uint32_t* __restrict__ steps_left) {
+      for (uint64_t r = blockIdx.x; r < n_rods; r += gridDim.x) {
+        if (steps_left && steps_left[r] == 0) continue;
+        const RodInfo ri = rods[r];
+        for (uint32_t k = threadIdx.x; k < ri.n; k += blockDim.x) reortho_slot(Q, E, ri.start + k);
       }
     }
 
@@ -601,6 +616,13 @@ namespace erod {
   int fused_reorthonormalize(erod_batch* b) {
     hipLaunchKernelGGL(k_reortho, dim3(grid_for(b->E)), dim3(kThreads), 0, b->stream, b->d_field[EROD_DIRECTOR], b->E);
     return hip_check(hipGetLastError(), "reorthonormalize");
+  }
+
+  int fused_reorthonormalize_live(erod_batch* b) {
+    const unsigned g = static_cast<unsigned>(b->n_rods < 65535 ? b->n_rods : 65535);
+    hipLaunchKernelGGL(k_reortho_live, dim3(g), dim3(256), 0, b->stream, b->d_field[EROD_DIRECTOR], b->E, b->d_rods,
+                       b->n_rods, static_cast<const uint32_t*>(b->d_steps_left));
+    return hip_check(hipGetLastError(), "scheduled reorthonormalization");
   }
 
   int fused_check_finite(erod_batch* b, uint8_t* flags) {

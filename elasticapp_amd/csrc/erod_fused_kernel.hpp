@@ -155,11 +155,13 @@ namespace erod {
     // FP32-bulk: the same map written in u itself,
     //   q' = q - th^2 B q + B (u . q) u + A (q x u),
     // A = sin(th)/th, B = (1 - cos(th))/th^2, both even series in th^2 = u . u
-    // (exact to rounding for th < 1e-2), so no sqrt or division per step.  Each
-    // column c of q (the components of d1, d2, d3 along one axis) maps as
-    // c' = M c with M = (1 - th^2 B) I + B u u^T + A [c -> c x u]: M is formed
-    // once (15 operations) and applied with 27 FMAs, against 57 operations for
-    // the three columns taken one at a time.
+    // (exact to rounding for th < 1e-2), so no sqrt or division per step;
+    // larger rotations per update (on the 4,096-rod ragged batch at each rod's
+    // stable dt, |w| dt stays below 5e-3) take sin and cos.  Each column c of
+    // q (the components of d1, d2, d3 along one axis) maps as c' = M c with
+    // M = (1 - th^2 B) I + B u u^T + A [c -> c x u]: M is formed once (15
+    // operations) and applied with 27 FMAs, against 57 operations for the
+    // three columns taken one at a time.
     template <>
     __device__ __forceinline__ void rodrigues<float>(double q[9], const double w[3], double cdt) {
       const double u0 = cdt * w[0], u1 = cdt * w[1], u2 = cdt * w[2];
@@ -170,12 +172,11 @@ namespace erod {
         A = __builtin_fma(t2, __builtin_fma(t2, __builtin_fma(t2, -1.0 / 5040.0, 1.0 / 120.0), -1.0 / 6.0), 1.0);
         B = __builtin_fma(t2, __builtin_fma(t2, __builtin_fma(t2, -1.0 / 40320.0, 1.0 / 720.0), -1.0 / 24.0), 0.5);
       } else {
-        const double th = sqrt(t2);
-        double s, c, sh, ch;
-        sincos(th, &s, &c);
-        sincos(0.5 * th, &sh, &ch);
-        A = s / th;
-        B = 2.0 * sh * sh / t2;
+        const double th = sqrt(t2), r = 1.0 / t2;
+        double s, c;
+        math::sincos(th, &s, &c);
+        A = s * th * r;
+        B = (1.0 - c) * r;
       }
       const double cq = __builtin_fma(-t2, B, 1.0);
       const double b0 = B * u0, b1 = B * u1, b2 = B * u2, a0 = A * u0, a1 = A * u1, a2 = A * u2;
@@ -284,17 +285,16 @@ namespace erod {
         K.J[c] = C.J[c * E + s];
         K.iJ[c] = C.iJ[c * E + s];
       }
-      if (has_vor) {
-        K.D0 = C.D0[s];
+      // (loads from slot 0 and selects at a rod's last element, whose Voronoi
+      // slot may lie past the arrays' end: no branch, and no stack copy of
+      // the constants that a branch over them leads the compiler to make)
+      const uint64_t sv = has_vor ? s : 0;
+      K.D0 = has_vor ? C.D0[sv] : R(1);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          K.kap0[c] = C.kap0[c * V + s];
-          K.B[c] = C.B[c * V + s];
-        }
-      } else {
-        K.D0 = R(1);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) K.kap0[c] = K.B[c] = R(0);
+      for (int c = 0; c < 3; ++c) {
+        const R k0 = C.kap0[c * V + sv], b = C.B[c * V + sv];
+        K.kap0[c] = has_vor ? k0 : R(0);
+        K.B[c] = has_vor ? b : R(0);
       }
       if constexpr (sizeof(R) == 4) {
         K.il0 = 1.0f / K.l0;
@@ -322,12 +322,14 @@ namespace erod {
                                                    const float dn[3], const double vk[3], const double vk1[3],
                                                    const double qk[9], const R qr[9], const double qk1[9],
                                                    const double wk[3], bool has_vor, const ElemConst<R>& K,
-                                                   Mid<R>& out) {
+                                                   Mid<R>& out, const float* geo = nullptr) {
       // spanwise geometry.  FP64: the reference's node difference, length and
       // tangent.  FP32-bulk: the node difference in FP64 rounded to FP32, then
       // everything in FP32; an element's length is a function of that rounded
       // difference alone, so the neighbour that needs it (for the Voronoi
-      // length) gets the same bits from the same difference (dn).
+      // length) gets the same bits from the same difference (dn).  `geo`, when
+      // given (FP32-bulk), holds that rounded difference, its length and the
+      // next element's length, formed once per element and step by the caller.
       R l, t[3];
       if constexpr (sizeof(R) == 8) {
         const double d0 = xk1[0] - xk[0], d1 = xk1[1] - xk[1], d2 = xk1[2] - xk[2];
@@ -336,9 +338,10 @@ namespace erod {
         t[1] = d1 / l;
         t[2] = d2 / l;
       } else {
-        const float d0 = static_cast<float>(xk1[0] - xk[0]), d1 = static_cast<float>(xk1[1] - xk[1]),
-                    d2 = static_cast<float>(xk1[2] - xk[2]);
-        l = len32(d0, d1, d2);
+        const float d0 = geo ? geo[0] : static_cast<float>(xk1[0] - xk[0]),
+                    d1 = geo ? geo[1] : static_cast<float>(xk1[1] - xk[1]),
+                    d2 = geo ? geo[2] : static_cast<float>(xk1[2] - xk[2]);
+        l = geo ? geo[3] : len32(d0, d1, d2);
         const float il = rcp_(l);
         t[0] = d0 * il;
         t[1] = d1 * il;
@@ -412,7 +415,7 @@ namespace erod {
           const double f0 = xk2[0] - xk1[0], f1 = xk2[1] - xk1[1], f2 = xk2[2] - xk1[2];
           l1 = len_<R>(f0, f1, f2);
         } else {
-          l1 = len32(dn[0], dn[1], dn[2]);
+          l1 = geo ? geo[4] : len32(dn[0], dn[1], dn[2]);
         }
         const R D = R(0.5) * (l + l1);
         R eps;
@@ -523,7 +526,7 @@ namespace erod {
       // last, the second node being the rod's tail)
       constexpr int kS1 = kCache ? 15 : 18;
       __shared__ double s1[kS1][NT];
-      __shared__ float s5[kCache ? 3 : 1][NT];
+      __shared__ float s5[kCache ? 4 : 1][NT];
       // exchange 2: per thread, its last element's c, P, R, in column tid + 1
       // (column 0 stays zero: the -1 terms of the tile's first element)
       __shared__ R s3[9][NT + 1];
@@ -542,13 +545,24 @@ namespace erod {
       uint64_t slot[P];
       double x[P][3], v[P][3], q[P][9], w[P][3];
       double tm[P], dt[P];
+      // per element: the rod's flags (bit1 end force, bit2 damping per node)
+      // with 1 + the clamp (0 or 1; 0: none) holding the element's node in
+      // bits 8-9, the element itself in bits 12-13, and, on the element
+      // holding the thread's tail node, that node in bits 10-11.  A clamp at
+      // index i holds node i and element min(i, n - 1).  Clamps are matched
+      // by the rod's own indices, so a piece applies those of its halo too.
       uint32_t rflags[P];
       uint32_t lim[P];  // steps this launch may take for the element's rod (its step budget)
       bool own[P];      // written back (false in a piece's halo)
       bool clockw[P];   // writes its rod's clock and budget
       bool tailw = false;  // the thread's tail node is its rod's last node (written back)
+      // damping rates of node and element j; FP32-bulk folds them, once the
+      // constants are in LDS, into m gf and (m + m_next)/2 gt (tail_gf into
+      // tail_m tail_gf)
       R gfj[P], gtj[P];
       R tail_m = R(0), tail_im = R(0), tail_gf = R(0);
+      // FP32-bulk: reciprocals of the element constants, formed once per launch
+      R inv_m[kCache ? P : 1], inv_l0[kCache ? P : 1], inv_D0[kCache ? P : 1], inv_J[kCache ? P : 1][3];
       double xt[3] = {0, 0, 0}, vt[3] = {0, 0, 0};
       int tailj = -1;
       // FP32-bulk without environment: the end force on the rod's last node
@@ -611,8 +625,17 @@ namespace erod {
         dt[j] = ri.dt;
         tm[j] = time_in[r];
         lim[j] = steps_in ? steps_in[r] : kNoBudget;
-        // the clamp acts at the rod's first element, the end force at its last node
-        rflags[j] = ri.flags & ((ends & kSegFirst) ? ~0u : ~1u) & ((ends & kSegLast) ? ~0u : ~2u);
+        // the end force acts at the rod's last node
+        rflags[j] = ri.flags & ~kClampBits & ((ends & kSegLast) ? ~0u : ~2u);
+        const uint32_t g = off + uint32_t(kk[j]);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          if (!valid[j] || !(ri.flags & clamp_bit(k))) continue;
+          const uint32_t i = ri.clamp_idx[k];
+          if (i == g) rflags[j] |= uint32_t(k + 1) << 8;
+          if (kk[j] == int(segn) - 1 && i == off + segn) rflags[j] |= uint32_t(k + 1) << 10;
+          if ((i < ri.n ? i : ri.n - 1) == g) rflags[j] |= uint32_t(k + 1) << 12;
+        }
         own[j] = valid[j] && uint32_t(kk[j]) >= w0 && uint32_t(kk[j]) < w1;
         clockw[j] = valid[j] && kk[j] == 0 && (segs == nullptr || (ends & kSegLast) != 0);
         if (valid[j] && kk[j] == int(segn) - 1) tailw = (ends & kSegLast) != 0;
@@ -690,6 +713,24 @@ namespace erod {
           }
         }
         __syncthreads();  // the constant table is written
+        // per element, for the launch: 1/m, 1/J, 1/l0, 1/D0 and the damping
+        // products m gf and (m + m_next)/2 gt (the same FP32 operations as
+        // forming them every step, so the same bits)
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          const uint32_t el = uint32_t(j) * NT + tid;                   // lane tid P + j
+          const uint32_t eln = j + 1 < P ? el + NT : uint32_t(tid) + 1;  // lane tid P + j + 1
+          const R mk = sc[0][el];
+          const R mkn = j == tailj ? tail_m : sc[0][eln];
+          inv_m[j] = rcp_(mk);
+          inv_l0[j] = rcp_(sc[1][el]);
+          inv_D0[j] = rcp_(sc[11][el]);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) inv_J[j][c] = rcp_(sc[8 + c][el]);
+          gfj[j] = mk * gfj[j];
+          gtj[j] = (R(0.5) * (mk + mkn)) * gtj[j];
+        }
+        tail_gf = tail_m * tail_gf;
       }
       if constexpr (!kCache) __syncthreads();  // every lookup in the rod table done before s1 is written
       if (tid < 9) s3[tid][0] = R(0);  // read only after the barriers of the first step
@@ -715,15 +756,36 @@ namespace erod {
               for (int c = 0; c < 3; ++c) xt[c] = maddd<R>(ch, vt[c], xt[c]);
           }
           tm[j] += 0.5 * dt[j];
-          if (kk[j] == 0 && (rflags[j] & 1u)) {
+          if (rflags[j] & (3u << 8)) {
             const RodInfo& ri = rods[rodi[j]];
+            int k = int((rflags[j] >> 8) & 3u) - 1;
+            // (k opaque here, so the pose's address is formed in this rarely
+            // taken branch instead of being kept in registers for the launch)
+            asm volatile("" : "+v"(k));
 #pragma unroll
-            for (int c = 0; c < 3; ++c) x[j][c] = ri.clamp_x[c];
+            for (int c = 0; c < 3; ++c) x[j][c] = ri.clamp[k].x[c];
+          }
+          if (rflags[j] & (3u << 12)) {
+            const RodInfo& ri = rods[rodi[j]];
+            int k = int((rflags[j] >> 12) & 3u) - 1;
+            asm volatile("" : "+v"(k));
 #pragma unroll
-            for (int m = 0; m < 9; ++m) q[j][m] = ri.clamp_q[m];
+            for (int m = 0; m < 9; ++m) q[j][m] = ri.clamp[k].q[m];
+          }
+          if (rflags[j] & (3u << 10)) {
+            const RodInfo& ri = rods[rodi[j]];
+            int k = int((rflags[j] >> 10) & 3u) - 1;
+            asm volatile("" : "+v"(k));
+#pragma unroll
+            for (int c = 0; c < 3; ++c) xt[c] = ri.clamp[k].x[c];
           }
         }
         // ---- exchange 1 -----------------------------------------------------------
+        // FP32-bulk with P > 1: per element, the FP32-rounded node difference
+        // and its length (geo[j] = {d, |d|, next element's |d|}), the first
+        // element's before the exchange, which publishes both
+        constexpr bool kGeo = kCache && P > 1;
+        float geo[kGeo ? P : 1][5];
         {
           const bool last0 = valid[0] && (kk[0] == nn[0] - 1);
 #pragma unroll
@@ -732,15 +794,36 @@ namespace erod {
             s1[3 + c][tid] = v[0][c];
             const double xs = last0 ? xt[c] : (P > 1 ? x[P > 1 ? 1 : 0][c] : 0.0);
             if constexpr (!kCache) s1[kCache ? 0 : 15 + c][tid] = xs;
-            else s5[kCache ? c : 0][tid] = static_cast<float>(xs - x[0][c]);
+            else {
+              geo[0][c] = static_cast<float>(xs - x[0][c]);
+              s5[kCache ? c : 0][tid] = geo[0][c];
+            }
+          }
+          if constexpr (kCache) {
+            geo[0][3] = len32(geo[0][0], geo[0][1], geo[0][2]);
+            s5[kCache ? 3 : 0][tid] = geo[0][3];
           }
 #pragma unroll
           for (int m = 0; m < 9; ++m) s1[6 + m][tid] = q[0][m];
         }
         __syncthreads();
-        Mid<R> mid[P];
         const int tn = tid + 1 < NT ? tid + 1 : tid;
         const int tn2 = tid + 2 < NT ? tid + 2 : tn;
+        if constexpr (kGeo) {
+#pragma unroll
+          for (int j = 1; j < P; ++j) {
+            const bool lastj = kk[j] == nn[j] - 1;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              const double xn = lastj ? xt[c] : (j + 1 < P ? x[j + 1 < P ? j + 1 : j][c] : s1[c][tn]);
+              geo[kGeo ? j : 0][c] = static_cast<float>(xn - x[j][c]);
+            }
+            geo[kGeo ? j : 0][3] = len32(geo[kGeo ? j : 0][0], geo[kGeo ? j : 0][1], geo[kGeo ? j : 0][2]);
+          }
+#pragma unroll
+          for (int j = 0; j < P; ++j) geo[kGeo ? j : 0][4] = j + 1 < P ? geo[kGeo ? (j + 1 < P ? j + 1 : j) : 0][3] : s5[kCache ? 3 : 0][tn];
+        }
+        Mid<R> mid[P];
 #pragma unroll
         for (int j = 0; j < P; ++j) {
           if (!valid[j]) {
@@ -829,15 +912,16 @@ namespace erod {
               K.B[c] = sc[kCache ? 15 + c : 0][kCache ? el : 0];
             }
             K.D0 = sc[kCache ? 11 : 0][kCache ? el : 0];
-            K.il0 = rcp_(K.l0);
-            K.iD0 = rcp_(K.D0);
+            K.il0 = inv_l0[kCache ? j : 0];
+            K.iD0 = inv_D0[kCache ? j : 0];
           } else {
             load_const<R>(C, slot[j], E, has_vor, K);
           }
           R qr[9];
 #pragma unroll
           for (int m = 0; m < 9; ++m) qr[m] = static_cast<R>(q[j][m]);
-          element_forces<R>(x[j], xk1, xk2, dn, v[j], vk1, q[j], qr, qk1, w[j], has_vor, K, mid[j]);
+          element_forces<R>(x[j], xk1, xk2, dn, v[j], vk1, q[j], qr, qk1, w[j], has_vor, K, mid[j],
+                            kGeo ? geo[kGeo ? j : 0] : nullptr);
         }
         // ---- exchange 2 -----------------------------------------------------------
 #pragma unroll
@@ -866,23 +950,24 @@ namespace erod {
           }
           const uint64_t s = slot[j];
           const Mid<R>& md = mid[j];
-          R mk, mkn, im, iJ[3];
+          R mk, im, iJ[3], mg, eg;
           if constexpr (kCache) {
-            const uint32_t el = uint32_t(j) * NT + tid;                   // lane tid P + j
-            const uint32_t eln = j + 1 < P ? el + NT : uint32_t(tid) + 1;  // lane tid P + j + 1
-            mk = sc[0][el];
-            mkn = last ? tail_m : sc[0][kCache ? eln : 0];
-            im = rcp_(mk);
+            // (the mass itself only for the environment's loads)
+            mk = kEnv != 0 ? sc[0][uint32_t(j) * NT + tid] : R(0);
+            im = inv_m[kCache ? j : 0];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) iJ[c] = rcp_(sc[kCache ? 8 + c : 0][kCache ? el : 0]);
+            for (int c = 0; c < 3; ++c) iJ[c] = inv_J[kCache ? j : 0][c];
+            mg = gfj[j];
+            eg = gtj[j];
           } else {
             mk = C.m[s];
-            mkn = C.m[s + 1];
+            const R mkn = C.m[s + 1];
             im = C.invm[s];
 #pragma unroll
             for (int c = 0; c < 3; ++c) iJ[c] = C.iJ[c * E + s];
+            mg = mk * gfj[j];
+            eg = (R(0.5) * (mk + mkn)) * gtj[j];
           }
-          const R gf = gfj[j], gt = gtj[j];
           // internal force on node k: difference of c, then damping
           // The rod-end cases of the difference and quadrature operators need no
           // branches: the -1 neighbour terms are zero at the first element and
@@ -892,12 +977,9 @@ namespace erod {
           // (madd is one FMA for FP32-bulk and, for FP64, a product and a sum
           // rounded on their own: the values of the reference's expressions)
           R vk[3] = {static_cast<R>(v[j][0]), static_cast<R>(v[j][1]), static_cast<R>(v[j][2])};
-          const R mg = mk * gf;
 #pragma unroll
           for (int c = 0; c < 3; ++c) f[c] = madd(-mg, vk[c], md.c[c] - cm[c]);
           // internal torque on element k
-          const R em = R(0.5) * (mk + mkn);
-          const R eg = em * gt;
           R tau[3];
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
@@ -921,7 +1003,7 @@ namespace erod {
           }
           // last node of the rod
           if (last) {
-            const R mgn = tail_m * tail_gf;
+            const R mgn = kCache ? tail_gf : tail_m * tail_gf;
             double fen[3] = {0.0, 0.0, 0.0};
             if (rflags[j] & 2u) {
               if constexpr (kTailForce) {
@@ -946,15 +1028,17 @@ namespace erod {
               const R a = (fn + static_cast<R>(fen[c])) * tail_im;
               vt[c] = maddd<R>(dtt, static_cast<double>(a), vt[c]);
             }
+            if (rflags[j] & (3u << 10))
+#pragma unroll
+              for (int c = 0; c < 3; ++c) vt[c] = 0.0;
           }
           // post-dynamic clamp
-          if (first && (rflags[j] & 1u)) {
+          if (rflags[j] & (3u << 8))
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-              v[j][c] = 0.0;
-              w[j][c] = 0.0;
-            }
-          }
+            for (int c = 0; c < 3; ++c) v[j][c] = 0.0;
+          if (rflags[j] & (3u << 12))
+#pragma unroll
+            for (int c = 0; c < 3; ++c) w[j][c] = 0.0;
         }
         // ---- time, kinematic half step ------------------------------------------
         // FP32-bulk: unless this is the rod's last step of the launch, also take

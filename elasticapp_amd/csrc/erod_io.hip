@@ -9,11 +9,14 @@
 // environment), so a run resumed from it continues bit for bit.
 //
 // Layout (little-endian, no padding):
-//   "ERODCKP1"  u32 version (2)  u32 damped  u64 n_rods  u64 element_slots
+//   "ERODCKP1"  u32 version (3)  u32 damped  u64 n_rods  u64 element_slots
 //   u64 n_elems[n_rods]  u64 n_active[n_rods]  u32 steps_left[n_rods]  u8 budgets
 //   f64 rod_time[n_rods]
-//   per rod: f64 dt, gamma_f, gamma_t  u32 flags  f64 clamp_x[3], clamp_q[9], force[3]
+//   per rod: f64 dt, gamma_f, gamma_t  u32 flags  f64 clamp 0 x[3], q[9]  f64 force[3]
 //            i32 ramp  f64 ramp_start, ramp_interval
+//            u32 clamp_idx[2]  f64 clamp 1 x[3], q[9]
+//   (version 1 and 2 files, without the last line, load with clamp 0 at index
+//   0 and no clamp 1)
 //   i32 gravity  f64 g[3]  i32 contact  f64 k, c, mu, ct, z0, offset  f64 n[3]
 //   (version 1 files, without the plane normal n, load with n = (0, 0, 1))
 //   u32 n_fields; per field: u32 id  u32 ncomp  u64 count  f64 data[ncomp * count]
@@ -31,7 +34,7 @@ using namespace erod;
 namespace {
   constexpr char kMagic[9] = "ERODCKP1";
   constexpr char kEnd[9] = "ERODEND1";
-  constexpr uint32_t kVersion = 2;
+  constexpr uint32_t kVersion = 3;
 
   struct File {
     std::FILE* f = nullptr;
@@ -70,9 +73,9 @@ int erod_save(erod_batch* b, const char* path) {
        put_n(f, b->steps_left.data(), b->n_rods) && put(f, uint8_t(b->budgets)) && put_n(f, t.data(), b->n_rods);
   for (uint64_t r = 0; ok && r < b->n_rods; ++r) {
     const RodInfo& ri = b->rods[r];
-    ok = put(f, ri.dt) && put(f, ri.gamma_f) && put(f, ri.gamma_t) && put(f, ri.flags) && put_n(f, ri.clamp_x, 3) &&
-         put_n(f, ri.clamp_q, 9) && put_n(f, ri.force, 3) && put(f, int32_t(ri.ramp)) && put(f, ri.ramp_start) &&
-         put(f, ri.ramp_interval);
+    ok = put(f, ri.dt) && put(f, ri.gamma_f) && put(f, ri.gamma_t) && put(f, ri.flags) && put_n(f, ri.clamp[0].x, 3) &&
+         put_n(f, ri.clamp[0].q, 9) && put_n(f, ri.force, 3) && put(f, int32_t(ri.ramp)) && put(f, ri.ramp_start) &&
+         put(f, ri.ramp_interval) && put_n(f, ri.clamp_idx, 2) && put_n(f, ri.clamp[1].x, 3) && put_n(f, ri.clamp[1].q, 9);
   }
   const Env& e = b->env;
   ok = ok && put(f, int32_t(e.gravity)) && put_n(f, e.g, 3) && put(f, int32_t(e.contact)) && put(f, e.k) && put(f, e.c) &&
@@ -103,7 +106,7 @@ int erod_load(const char* path, erod_batch** out) {
   uint64_t n_rods = 0, E = 0;
   if (!get_n(f, magic, 8) || std::memcmp(magic, kMagic, 8) != 0 || !get(f, version) || version < 1 ||
       version > kVersion || !get(f, damped) || !get(f, n_rods) || !get(f, E) || n_rods == 0 || n_rods > UINT32_MAX)
-    return io_error(std::string("erod_load: not an erod checkpoint (version 1 or 2): ") + path);
+    return io_error(std::string("erod_load: not an erod checkpoint (version 1 to 3): ") + path);
   // every rod takes at least its header entries and its record: refuse a count
   // the file cannot hold before sizing anything by it
   constexpr uint64_t kRodBytes = 8 + 8 + 4 + 8 + 3 * 8 + 4 + 15 * 8 + 4 + 2 * 8;
@@ -126,12 +129,20 @@ int erod_load(const char* path, erod_batch** out) {
   for (uint64_t r = 0; ok && r < n_rods; ++r) {
     RodInfo& ri = b->rods[r];
     int32_t ramp = 0;
-    ok = get(f, ri.dt) && get(f, ri.gamma_f) && get(f, ri.gamma_t) && get(f, ri.flags) && get_n(f, ri.clamp_x, 3) &&
-         get_n(f, ri.clamp_q, 9) && get_n(f, ri.force, 3) && get(f, ramp) && get(f, ri.ramp_start) && get(f, ri.ramp_interval);
+    ok = get(f, ri.dt) && get(f, ri.gamma_f) && get(f, ri.gamma_t) && get(f, ri.flags) && get_n(f, ri.clamp[0].x, 3) &&
+         get_n(f, ri.clamp[0].q, 9) && get_n(f, ri.force, 3) && get(f, ramp) && get(f, ri.ramp_start) && get(f, ri.ramp_interval);
+    if (version >= 3) {
+      ok = ok && get_n(f, ri.clamp_idx, 2) && get_n(f, ri.clamp[1].x, 3) && get_n(f, ri.clamp[1].q, 9);
+    } else {
+      ri.clamp_idx[0] = ri.clamp_idx[1] = 0;
+      ri.flags &= ~clamp_bit(1);
+    }
     ri.ramp = ramp;
     if (n_active[r] > n_elems[r] || n_active[r] == 1) ok = false;
     ri.n = static_cast<uint32_t>(n_active[r]);
-    b->has_clamps |= (ri.flags & 1u) != 0;
+    for (int k = 0; k < 2; ++k)
+      if ((ri.flags & clamp_bit(k)) && ri.clamp_idx[k] > ri.n) ok = false;
+    b->has_clamps |= (ri.flags & kClampBits) != 0;
     b->has_forces |= (ri.flags & 2u) != 0;
   }
   Env& e = b->env;

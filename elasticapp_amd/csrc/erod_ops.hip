@@ -299,27 +299,39 @@ namespace erod {
 
     // ---- hooks ---------------------------------------------------------------
 
+    // ConstrainPrimitives of a fixed clamp at index i: node i's position and
+    // element min(i, n - 1)'s director are held
     __global__ void k_clamp_values(double* __restrict__ x, double* __restrict__ Q, const RodInfo* __restrict__ rods,
                                    uint64_t n_rods, uint64_t E) {
       const uint64_t N = E + 1;
       GRID_STRIDE(r, n_rods) {
         const RodInfo& ri = rods[r];
-        if (!(ri.flags & 1u)) continue;
-        const uint64_t s = ri.start;
-        for (int d = 0; d < 3; ++d) x[d * N + s] = ri.clamp_x[d];
-        for (int m = 0; m < 9; ++m) Q[m * E + s] = ri.clamp_q[m];
+        for (int k = 0; k < 2; ++k) {
+          const uint32_t i = ri.clamp_idx[k];
+          if (!(ri.flags & clamp_bit(k)) || i > ri.n) continue;
+          const uint64_t s = ri.start + i;
+          const uint64_t se = ri.start + (i < ri.n ? i : ri.n - 1);
+          for (int d = 0; d < 3; ++d) x[d * N + s] = ri.clamp[k].x[d];
+          for (int m = 0; m < 9; ++m) Q[m * E + se] = ri.clamp[k].q[m];
+        }
       }
     }
 
+    // ConstrainRates of the same clamp: its node's velocity and its
+    // element's angular velocity are zeroed
     __global__ void k_clamp_rates(double* __restrict__ v, double* __restrict__ w, const RodInfo* __restrict__ rods,
                                   uint64_t n_rods, uint64_t E) {
       const uint64_t N = E + 1;
       GRID_STRIDE(r, n_rods) {
         const RodInfo& ri = rods[r];
-        if (!(ri.flags & 1u)) continue;
-        const uint64_t s = ri.start;
-        for (int d = 0; d < 3; ++d) v[d * N + s] = 0.0;
-        for (int d = 0; d < 3; ++d) w[d * E + s] = 0.0;
+        for (int k = 0; k < 2; ++k) {
+          const uint32_t i = ri.clamp_idx[k];
+          if (!(ri.flags & clamp_bit(k)) || i > ri.n) continue;
+          const uint64_t s = ri.start + i;
+          const uint64_t se = ri.start + (i < ri.n ? i : ri.n - 1);
+          for (int d = 0; d < 3; ++d) v[d * N + s] = 0.0;
+          for (int d = 0; d < 3; ++d) w[d * E + se] = 0.0;
+        }
       }
     }
 

@@ -3,10 +3,16 @@
 // A server owns a pool batch of rod slots in size classes (slot capacities in
 // elements).  A request names rods of a template batch (the library) and a
 // number of steps.  Each tick
-//   1. admits queued requests first-come first-served while every rod of the
-//      head request finds a free slot (the smallest class that holds it):
-//      one device copy puts the template rods into their slots and each slot
-//      gets the request's steps as its step budget;
+//   1. admits queued requests in order: a request whose rods all find free
+//      slots (each the smallest class that holds it) is admitted; one device
+//      copy puts the template rods into their slots and each slot gets the
+//      request's steps as its step budget.  A server created with a budget
+//      only (erod_server_create_budgeted) first grows the pool for the
+//      earliest waiting request, a class at a time by doubling, as far as
+//      the HBM budget allows.  The earliest request that still waits reserves
+//      every class it needs: later requests are admitted past it only into
+//      slots of the other classes, so it waits for nothing but slots of its
+//      own classes to be released;
 //   2. advances every slot with budget left by up to `steps_per_tick` fused
 //      steps (rods whose budget runs out stop mid-launch and keep their state);
 //   3. waits for the device and reports the requests whose rods have all spent
@@ -110,6 +116,9 @@ struct erod_server {
   int precision = EROD_FP32_BULK;
   int steps_per_tick = 100;
   uint64_t pool_bytes = 0;
+  double hbm_budget = 0.0;
+  bool grow = false;      // the pool grows on demand within hbm_budget
+  uint64_t growths = 0;   // pool rebuilds so far
   std::vector<uint64_t> class_capacity, class_total;
   std::vector<std::vector<uint64_t>> free_slots;  // per class, a stack of free slot ids
   std::vector<int> slot_class;
@@ -127,6 +136,20 @@ struct erod_server {
 };
 
 namespace {
+  // HBM of a pool of slots with these capacities: every field over every
+  // slot, the FP32 constant copy, rod records
+  double pool_bytes_for(const std::vector<uint64_t>& n_elems) {
+    uint64_t slots = 0;
+    for (auto n : n_elems) slots += n + 2;
+    double bytes = 0.0;
+    for (int f = 0; f < EROD_N_FIELDS; ++f) bytes += 8.0 * field_info(f).ncomp * double(slots);
+    return bytes + 4.0 * 22.0 * double(slots) + double(sizeof(RodInfo) + 64) * double(n_elems.size());
+  }
+
+  // the pool's slots as a fresh batch: every slot empty with a zero budget,
+  // in the library's environment, stepping steps_per_tick steps per launch
+  int make_pool(erod_server* s, const std::vector<uint64_t>& n_elems, erod_batch** out);
+
   int check_server(const erod_server* s) {
     if (!s) {
       set_error("null server");
@@ -135,19 +158,37 @@ namespace {
     return EROD_OK;
   }
 
-  // take a slot of the smallest class holding n elements; -1 if none is free
-  int64_t take_slot(erod_server* s, uint64_t n) {
-    for (size_t c = 0; c < s->class_capacity.size(); ++c)
-      if (s->class_capacity[c] >= n && !s->free_slots[c].empty()) {
-        const uint64_t slot = s->free_slots[c].back();
-        s->free_slots[c].pop_back();
-        return static_cast<int64_t>(slot);
-      }
-    return -1;
-  }
-
   void give_back(erod_server* s, uint64_t slot) { s->free_slots[s->slot_class[slot]].push_back(slot); }
 
+  // the smallest class holding n elements (the last class if none does)
+  size_t class_of(const erod_server* s, uint64_t n) {
+    size_t c = 0;
+    while (c + 1 < s->class_capacity.size() && s->class_capacity[c] < n) ++c;
+    return c;
+  }
+
+  // take slots for every rod of a request, each of the smallest class that
+  // holds it and is not reserved; all or nothing
+  bool place(erod_server* s, const std::vector<uint64_t>& lib_rods, const std::vector<char>& reserved,
+             std::vector<uint64_t>& got) {
+    got.clear();
+    for (uint64_t r : lib_rods) {
+      const uint64_t n = s->library->n_active[r];
+      int64_t slot = -1;
+      for (size_t c = 0; c < s->class_capacity.size() && slot < 0; ++c)
+        if (!reserved[c] && s->class_capacity[c] >= n && !s->free_slots[c].empty()) {
+          slot = static_cast<int64_t>(s->free_slots[c].back());
+          s->free_slots[c].pop_back();
+        }
+      if (slot < 0) {
+        for (uint64_t g : got) give_back(s, g);
+        got.clear();
+        return false;
+      }
+      got.push_back(static_cast<uint64_t>(slot));
+    }
+    return true;
+  }
   // empty a slot without touching the pool's damping bookkeeping (the
   // emptied slot is skipped by it)
   void empty_slot(erod_batch* b, uint64_t slot) {
@@ -160,14 +201,118 @@ namespace {
     b->plan.valid = false;
     b->graph_precision = -1;
   }
+
+  int make_pool(erod_server* s, const std::vector<uint64_t>& n_elems, erod_batch** out) {
+    erod_batch* p = nullptr;
+    int rc = erod_batch_create(n_elems.data(), n_elems.size(), s->library->damped ? 1 : 0, &p);
+    if (rc) return rc;
+    if ((rc = erod_set_rod_step_budget(p, 0, 0))) {
+      erod_batch_destroy(p);
+      return rc;
+    }
+    for (uint64_t r = 0; r < p->n_rods; ++r) empty_slot(p, r);
+    p->all_full = false;
+    p->damping_dirty = false;  // empty slots carry no damping; copied rods bring their rates
+    p->env = s->library->env;
+    p->plan.steps_per_launch = s->steps_per_tick;
+    *out = p;
+    return EROD_OK;
+  }
+
+  // grow the pool by add[c] slots of each class: a new pool batch holding the
+  // old slots under the same ids (the rods in them copied bit for bit, with
+  // their clocks, records and budgets) and the new slots after them
+  int grow_pool(erod_server* s, const std::vector<uint64_t>& add) {
+    erod_batch* old = s->pool;
+    std::vector<uint64_t> n_elems = old->n_elems;
+    std::vector<int> cls;
+    for (size_t c = 0; c < add.size(); ++c)
+      for (uint64_t i = 0; i < add[c]; ++i) {
+        n_elems.push_back(s->class_capacity[c]);
+        cls.push_back(static_cast<int>(c));
+      }
+    erod_batch* p = nullptr;
+    int rc = make_pool(s, n_elems, &p);
+    if (rc) return rc;
+    // the tile configuration the caller set on the pool carries over
+    p->plan.lanes_per_tile = old->plan.lanes_per_tile;
+    p->plan.elements_per_thread = old->plan.elements_per_thread;
+    p->plan.steps_per_launch = old->plan.steps_per_launch;
+    p->plan.split_lanes = old->plan.split_lanes;
+    p->plan.split_epl = old->plan.split_epl;
+    p->plan.seg_lanes = old->plan.seg_lanes;
+    p->plan.seg_epl = old->plan.seg_epl;
+    p->plan.halo = old->plan.halo;
+    // (requests admitted in this tick but not yet copied hold slots whose
+    // rods arrive after the growth)
+    std::vector<uint64_t> used;
+    for (auto& kv : s->requests)
+      if (kv.second->running || kv.second->done)
+        for (uint64_t slot : kv.second->slots) used.push_back(slot);
+    if (!used.empty()) {
+      if ((rc = erod_copy_rods(p, used.data(), old, used.data(), used.size()))) {
+        erod_batch_destroy(p);
+        return rc;
+      }
+      for (uint64_t slot : used) p->steps_left[slot] = old->steps_left[slot];
+      p->budgets_dirty = true;
+    }
+    erod_batch_destroy(old);
+    s->pool = p;
+    const uint64_t first = s->slot_class.size();
+    for (size_t i = 0; i < cls.size(); ++i) {
+      s->slot_class.push_back(cls[i]);
+      s->free_slots[cls[i]].push_back(first + i);
+    }
+    for (size_t c = 0; c < add.size(); ++c) {
+      s->class_total[c] += add[c];
+      // (lowest ids on top of the stack)
+      std::sort(s->free_slots[c].rbegin(), s->free_slots[c].rend());
+    }
+    s->pool_bytes = static_cast<uint64_t>(pool_bytes_for(n_elems));
+    ++s->growths;
+    return EROD_OK;
+  }
+
+  // grow the pool so that the request fits it, within the HBM budget: each
+  // class that lacks slots doubles, or grows by what it lacks if that is
+  // more, or, when doubling would pass the budget, by what it lacks alone
+  int grow_for(erod_server* s, const std::vector<uint64_t>& lib_rods, bool* grown) {
+    *grown = false;
+    std::vector<uint64_t> need(s->class_capacity.size(), 0), add(need.size(), 0);
+    for (uint64_t r : lib_rods) ++need[class_of(s, s->library->n_active[r])];
+    bool short_of = false;
+    for (size_t c = 0; c < need.size(); ++c)
+      if (need[c] > s->free_slots[c].size()) {
+        const uint64_t lack = need[c] - s->free_slots[c].size();
+        add[c] = std::max(lack, s->class_total[c]);
+        short_of = true;
+      }
+    if (!short_of) return EROD_OK;
+    auto bytes_with = [&](const std::vector<uint64_t>& a) {
+      std::vector<uint64_t> n = s->pool->n_elems;
+      for (size_t c = 0; c < a.size(); ++c) n.insert(n.end(), a[c], s->class_capacity[c]);
+      return pool_bytes_for(n);
+    };
+    if (bytes_with(add) > s->hbm_budget) {
+      for (size_t c = 0; c < need.size(); ++c)
+        add[c] = need[c] > s->free_slots[c].size() ? need[c] - s->free_slots[c].size() : 0;
+      if (bytes_with(add) > s->hbm_budget) return EROD_OK;  // waits for slots to be released
+    }
+    int rc = grow_pool(s, add);
+    if (rc == EROD_OK) *grown = true;
+    return rc;
+  }
+
 }  // namespace
 
 extern "C" {
 
-int erod_server_create(erod_batch* library, const uint64_t* class_capacity, const uint64_t* class_slots, int n_classes,
-                       int precision, int steps_per_tick, double hbm_budget_bytes, erod_server** out) {
-  if (!library || !class_capacity || !class_slots || n_classes <= 0 || steps_per_tick <= 0 || !out ||
-      (precision != EROD_FP64 && precision != EROD_FP32_BULK)) {
+// the server of both create calls; class_slots: the initial slot count of each class
+static int server_create(erod_batch* library, const uint64_t* class_capacity, const uint64_t* class_slots, int n_classes,
+                         int precision, int steps_per_tick, double hbm_budget_bytes, bool grow, erod_server** out) {
+  if (!library || !class_capacity || n_classes <= 0 || steps_per_tick <= 0 || !out ||
+      (precision != EROD_FP64 && precision != EROD_FP32_BULK) || !(hbm_budget_bytes > 0.0)) {
     set_error("erod_server_create: bad argument");
     return EROD_ERR_INVALID;
   }
@@ -181,14 +326,14 @@ int erod_server_create(erod_batch* library, const uint64_t* class_capacity, cons
     cls.emplace_back(class_capacity[c], class_slots[c]);
   }
   std::sort(cls.begin(), cls.end());
+  for (size_t c = 1; c < cls.size(); ++c)
+    if (cls[c].first == cls[c - 1].first) {
+      set_error("erod_server_create: two classes of one capacity");
+      return EROD_ERR_INVALID;
+    }
   std::vector<uint64_t> n_elems;
   for (auto& [cap, cnt] : cls) n_elems.insert(n_elems.end(), cnt, cap);
-  // HBM of the pool: every field over every slot, the FP32 constant copy, rod records
-  uint64_t slots = 0;
-  for (auto n : n_elems) slots += n + 2;
-  double bytes = 0.0;
-  for (int f = 0; f < EROD_N_FIELDS; ++f) bytes += 8.0 * field_info(f).ncomp * double(slots);
-  bytes += 4.0 * 22.0 * double(slots) + double(sizeof(RodInfo) + 64) * double(n_elems.size());
+  const double bytes = pool_bytes_for(n_elems);
   if (bytes > hbm_budget_bytes) {
     set_error("erod_server_create: the pool (" + std::to_string(bytes / 1e9) + " GB) exceeds the HBM budget");
     return EROD_ERR_INVALID;
@@ -198,9 +343,10 @@ int erod_server_create(erod_batch* library, const uint64_t* class_capacity, cons
   s->precision = precision;
   s->steps_per_tick = steps_per_tick;
   s->pool_bytes = static_cast<uint64_t>(bytes);
-  int rc = erod_batch_create(n_elems.data(), n_elems.size(), library->damped ? 1 : 0, &s->pool);
+  s->hbm_budget = hbm_budget_bytes;
+  s->grow = grow;
+  int rc = make_pool(s.get(), n_elems, &s->pool);
   if (rc) return rc;
-  erod_batch* p = s->pool;
   uint64_t slot = 0;
   for (size_t c = 0; c < cls.size(); ++c) {
     s->class_capacity.push_back(cls[c].first);
@@ -212,18 +358,30 @@ int erod_server_create(erod_batch* library, const uint64_t* class_capacity, cons
     }
     std::reverse(s->free_slots[c].begin(), s->free_slots[c].end());
   }
-  // every slot starts empty, with a zero budget; each tick takes the library's environment
-  if ((rc = erod_set_rod_step_budget(p, 0, 0))) {
-    erod_batch_destroy(p);
-    return rc;
-  }
-  for (uint64_t r = 0; r < p->n_rods; ++r) empty_slot(p, r);
-  p->all_full = false;
-  p->damping_dirty = false;  // empty slots carry no damping; copied rods bring their rates
-  p->env = library->env;
-  p->plan.steps_per_launch = steps_per_tick;
   *out = s.release();
   return EROD_OK;
+}
+
+int erod_server_create(erod_batch* library, const uint64_t* class_capacity, const uint64_t* class_slots, int n_classes,
+                       int precision, int steps_per_tick, double hbm_budget_bytes, erod_server** out) {
+  if (!class_slots) {
+    set_error("erod_server_create: bad argument");
+    return EROD_ERR_INVALID;
+  }
+  return server_create(library, class_capacity, class_slots, n_classes, precision, steps_per_tick, hbm_budget_bytes,
+                       false, out);
+}
+
+int erod_server_create_budgeted(erod_batch* library, const uint64_t* class_capacity, int n_classes, int precision,
+                                int steps_per_tick, double hbm_budget_bytes, erod_server** out) {
+  if (n_classes <= 0) {
+    set_error("erod_server_create_budgeted: bad argument");
+    return EROD_ERR_INVALID;
+  }
+  // a few slots of each class to start with; the pool grows as requests come
+  const std::vector<uint64_t> initial(static_cast<size_t>(n_classes), 16);
+  return server_create(library, class_capacity, initial.data(), n_classes, precision, steps_per_tick, hbm_budget_bytes,
+                       true, out);
 }
 
 int erod_server_destroy(erod_server* s) {
@@ -261,20 +419,29 @@ int erod_server_submit(erod_server* s, uint64_t id, const uint64_t* library_rods
     rq->lib_rods.push_back(r);
     rq->elements += s->library->n_active[r];
   }
-  // a request must fit the empty pool, or it would block the queue forever
+  // a request must fit the empty pool (one that may grow: grown as far as the
+  // budget allows), or it would wait forever
   std::vector<uint64_t> need(rq->lib_rods.size());
   for (size_t i = 0; i < need.size(); ++i) need[i] = s->library->n_active[rq->lib_rods[i]];
   std::sort(need.rbegin(), need.rend());
   std::vector<uint64_t> left(s->class_capacity.size());
   for (size_t c = 0; c < left.size(); ++c) left[c] = s->class_total[c];
+  std::vector<uint64_t> grown = s->pool->n_elems;
   for (uint64_t n : need) {
     size_t c = 0;
     while (c < left.size() && (s->class_capacity[c] < n || left[c] == 0)) ++c;
-    if (c == left.size()) {
+    if (c < left.size()) {
+      --left[c];
+    } else if (s->grow) {
+      grown.push_back(s->class_capacity[class_of(s, n)]);
+    } else {
       set_error("erod_server_submit: the request does not fit the pool");
       return EROD_ERR_INVALID;
     }
-    --left[c];
+  }
+  if (s->grow && pool_bytes_for(grown) > s->hbm_budget) {
+    set_error("erod_server_submit: the request does not fit the pool within the HBM budget");
+    return EROD_ERR_INVALID;
   }
   s->queue.push_back(rq.get());
   s->requests[id] = std::move(rq);
@@ -283,28 +450,46 @@ int erod_server_submit(erod_server* s, uint64_t id, const uint64_t* library_rods
 
 int erod_server_tick(erod_server* s, uint64_t* done_ids, uint64_t capacity, uint64_t* n_done) {
   if (check_server(s) || !n_done || (capacity && !done_ids)) return EROD_ERR_INVALID;
-  erod_batch* p = s->pool;
-  // 1. admission, first come first served
+  // 1. admission, in queue order: the earliest request that cannot be placed
+  // (after growing the pool, if it may grow) reserves the classes it needs;
+  // up to kLookahead later requests are tried past it, outside those classes
+  constexpr size_t kLookahead = 1024;
   std::vector<uint64_t> dst, src;
   std::vector<erod_server::Request*> admitted;
-  while (!s->queue.empty()) {
-    erod_server::Request* rq = s->queue.front();
+  std::vector<char> reserved(s->class_capacity.size(), 0);
+  bool blocked = false;
+  size_t past = 0;
+  for (auto it = s->queue.begin(); it != s->queue.end() && past < kLookahead;) {
+    erod_server::Request* rq = *it;
     std::vector<uint64_t> got;
-    for (uint64_t r : rq->lib_rods) {
-      const int64_t slot = take_slot(s, s->library->n_active[r]);
-      if (slot < 0) break;
-      got.push_back(static_cast<uint64_t>(slot));
+    bool ok = place(s, rq->lib_rods, reserved, got);
+    if (!ok && !blocked && s->grow) {
+      bool grown = false;
+      if (int grc = grow_for(s, rq->lib_rods, &grown)) {
+        for (erod_server::Request* a : admitted) {
+          for (uint64_t slot : a->slots) give_back(s, slot);
+          a->slots.clear();
+        }
+        return grc;
+      }
+      if (grown) ok = place(s, rq->lib_rods, reserved, got);
     }
-    if (got.size() < rq->lib_rods.size()) {
-      for (uint64_t slot : got) give_back(s, slot);
-      break;
+    if (ok) {
+      rq->slots = got;
+      dst.insert(dst.end(), got.begin(), got.end());
+      src.insert(src.end(), rq->lib_rods.begin(), rq->lib_rods.end());
+      admitted.push_back(rq);
+      it = s->queue.erase(it);
+    } else {
+      if (!blocked) {
+        blocked = true;
+        for (uint64_t r : rq->lib_rods) reserved[class_of(s, s->library->n_active[r])] = 1;
+      }
+      ++it;
     }
-    rq->slots = got;
-    dst.insert(dst.end(), got.begin(), got.end());
-    src.insert(src.end(), rq->lib_rods.begin(), rq->lib_rods.end());
-    admitted.push_back(rq);
-    s->queue.pop_front();
+    if (blocked) ++past;
   }
+  erod_batch* p = s->pool;
   int rc = EROD_OK;
   if (!dst.empty()) {
     rc = erod_copy_rods(p, dst.data(), s->library, src.data(), dst.size());
@@ -400,6 +585,14 @@ int erod_server_status(const erod_server* s, uint64_t* queued, uint64_t* running
   if (running) *running = s->running.size();
   if (active_elements) *active_elements = s->active_elements;
   if (pool_bytes) *pool_bytes = s->pool_bytes;
+  return EROD_OK;
+}
+
+int erod_server_pool_info(const erod_server* s, uint64_t* slots, uint64_t* growths, double* hbm_budget_bytes) {
+  if (check_server(s)) return EROD_ERR_INVALID;
+  if (slots) *slots = s->slot_class.size();
+  if (growths) *growths = s->growths;
+  if (hbm_budget_bytes) *hbm_budget_bytes = s->hbm_budget;
   return EROD_OK;
 }
 

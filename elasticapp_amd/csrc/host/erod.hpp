@@ -112,7 +112,10 @@ namespace erod {
     void dynamic_update(double dt) { check(erod_dynamic_update(b_, dt)); }
 
     // module hooks as device descriptors
-    void clamp(std::size_t rod, const double x[3], const double Q[9]) { check(erod_add_clamp(b_, rod, 0, x, Q)); }
+    // node `index` (and element `index`, below n) held at x and Q
+    void clamp(std::size_t rod, const double x[3], const double Q[9], std::size_t index = 0) {
+      check(erod_add_clamp(b_, rod, index, x, Q));
+    }
     void end_force(std::size_t rod, const double F[3], erod_ramp ramp = EROD_RAMP_NONE, double start = 0.0,
                    double interval = 1.0) {
       check(erod_add_end_force(b_, rod, F, ramp, start, interval));

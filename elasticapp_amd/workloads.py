@@ -48,7 +48,7 @@ class Workload:
     states: list
     dt: float
     steps: int
-    clamps: list = field(default_factory=list)   # (rod, x[3], Q[9])
+    clamps: list = field(default_factory=list)   # (rod, x[3], Q[9]) or (rod, x[3], Q[9], index)
     forces: list = field(default_factory=list)   # (rod, F[3], ramp, start, interval)
     gravity: tuple = None
     contact: tuple = None                        # (k, c, mu, ct, z0, offset[, normal[3]])
@@ -246,8 +246,8 @@ def serving_requests(n_requests, n_templates, seed=3004, max_rods=64, min_steps=
 
 def apply_hooks(target, wl):
     """Register the workload's hooks on a DeviceBlock or an oracle System."""
-    for rod, x, Q in wl.clamps:
-        target.add_clamp(rod, x, Q)
+    for rod, x, Q, *index in wl.clamps:
+        target.add_clamp(rod, x, Q, index[0] if index else 0)
     for rod, F, ramp, start, interval in wl.forces:
         target.add_end_force(rod, F, ramp, start, interval)
     if wl.gravity is not None:

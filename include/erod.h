@@ -79,6 +79,10 @@ int erod_batch_destroy(erod_batch* b);
 int erod_batch_device(const erod_batch* b, int* device);
 /* Total element slots (ghosts included); nodes = slots + 1, voronoi = slots - 1. */
 int erod_batch_slots(const erod_batch* b, uint64_t* element_slots);
+/* Number of rods (slots), and, when `capacity` is not null, each rod's slot
+ * capacity in elements (n_rods entries; Block::emplace's n of each system,
+ * Block.hpp:568-611). */
+int erod_batch_rods(const erod_batch* b, uint64_t* n_rods, uint64_t* capacity);
 /* First slot of a rod (Block::get_start_of_slice, Block.hpp:216-227). */
 int erod_rod_start(const erod_batch* b, uint64_t rod, uint64_t* start);
 /* Lines and stagger of a field. */
@@ -118,12 +122,18 @@ int erod_compute_internal_dynamics(erod_batch* b, double time);
 int erod_dynamic_update(erod_batch* b, double dt);
 
 /* Device-side hook descriptors that replace per-step host lambdas.
- * Clamp: value constraint after the kinematic half step (position of node
- * `index`, director of element `index`), rate constraint after the dynamic
- * step (ConstrainPrimitives / ConstrainRates,
- * elastica/Simulator/Interface/Constraints/ExecutionPolicy.hpp:591-593).
- * Only index 0 (the rod's first node and element, as in the reference's
- * examples) is supported; other indices return EROD_ERR_UNSUPPORTED. */
+ * Clamp: value constraint after the kinematic half step (position of a node,
+ * director of an element), rate constraint after the dynamic
+ * step (velocity and angular velocity zeroed); the constraint's per-index
+ * constrain_primitives / constrain_rates (ConstrainPrimitives /
+ * ConstrainRates, elastica/Simulator/Interface/Constraints/ExecutionPolicy.hpp:591-593,
+ * elastica/Simulator/Constraints/Constraint.hpp:290-313).  index runs from 0
+ * to n (the rod's active length) and holds node `index` with element
+ * min(index, n - 1): index n clamps the rod's last end as index 0 clamps its
+ * first.  A rod takes up to two clamps at different indices (a clamp at an
+ * index that has one replaces it; clamps at n - 1 and n, which would both
+ * hold element n - 1, are refused); a third returns EROD_ERR_UNSUPPORTED, an
+ * index past n EROD_ERR_INVALID. */
 int erod_add_clamp(erod_batch* b, uint64_t rod, uint64_t index, const double x[3], const double Q[9]);
 /* ExternalLoads(:, last node) = F * ramp(t) before the dynamic step
  * (examples/timoshenko/TimoshenkoCaseStudy.cpp:128-141; ramps
@@ -209,6 +219,18 @@ int erod_step_fused_graph(erod_batch* b, int precision, uint64_t n, uint64_t chu
 int erod_capture_fused_graph(erod_batch* b, int precision, uint64_t chunk);
 /* Re-orthonormalise every director (one Newton-Schulz polar step, FP64). */
 int erod_reorthonormalize(erod_batch* b);
+/* Schedule the same step on the fused paths (erod_step_fused,
+ * erod_step_fused_graph, and inside their captured graphs): after the launch
+ * during which the batch's fused step count, counted from this call, reaches
+ * a multiple of `every`, the directors of every rod with steps left (no step
+ * budget, or one not yet spent) are re-orthonormalised; rods whose budget is
+ * spent keep their state.  every = 0 (the default) schedules nothing.  A
+ * graph chunk that is a multiple of `every` replays one captured schedule;
+ * other chunks capture again whenever a replay starts at another phase.
+ * No reference counterpart: its directors are never re-orthonormalised. */
+int erod_set_reortho_interval(erod_batch* b, uint64_t every);
+/* The interval and the steps taken since the last scheduled step. */
+int erod_get_reortho_interval(const erod_batch* b, uint64_t* every, uint64_t* phase);
 /* Average duration (ms) of one launch of the fused step kernel over the last
  * erod_step_fused / erod_step_fused_graph call: HIP events recorded on the
  * batch stream before the first and after the last launch, divided by the
@@ -245,15 +267,27 @@ int erod_copy_rods(erod_batch* dst, const uint64_t* dst_rods, erod_batch* src, c
 /* A server owns a pool batch of empty rod slots in size classes (n_classes
  * capacities and slot counts) and serves requests made of rods of `library`
  * (which must outlive it).  The pool's HBM (all fields of all slots) must fit
- * `hbm_budget_bytes`.  Every tick admits queued requests first come first
- * served (all rods of a request at once, each into the smallest free slot that
- * holds it), advances every slot with budget left by up to `steps_per_tick`
- * fused steps at `precision`, waits for the device, and reports finished
- * requests.  No counterpart in the reference; its nearest analogue is the
- * first-fit blocking of rods at emplace time (BlockingPolicy.hpp:202-222). */
+ * `hbm_budget_bytes`.  Every tick admits queued requests in order (all rods
+ * of a request at once, each into the smallest free slot that holds it; the
+ * earliest request that does not fit reserves the classes it needs, and later
+ * ones are admitted past it into slots of other classes only), advances every
+ * slot with budget left by up to `steps_per_tick` fused steps at `precision`,
+ * waits for the device, and reports finished requests.  No counterpart in
+ * the reference; its nearest analogue is the first-fit blocking of rods at
+ * emplace time (BlockingPolicy.hpp:202-222). */
 typedef struct erod_server erod_server;
 int erod_server_create(erod_batch* library, const uint64_t* class_capacity, const uint64_t* class_slots, int n_classes,
                        int precision, int steps_per_tick, double hbm_budget_bytes, erod_server** out);
+/* Same, admitting against the HBM budget: the pool starts with a few slots
+ * of each class and, when the earliest waiting request finds no room, grows
+ * (each class it lacks slots of doubles, or grows by what it lacks, or by
+ * what it lacks alone when doubling would pass the budget) while the pool
+ * stays within `hbm_budget_bytes`.  Growing rebuilds the pool batch: the rods
+ * in it are copied bit for bit under the same slot ids, so results do not
+ * change, and a pool batch taken from erod_server_pool before a tick may be
+ * gone after it.  During a growth the old and the new pool are both held. */
+int erod_server_create_budgeted(erod_batch* library, const uint64_t* class_capacity, int n_classes, int precision,
+                                int steps_per_tick, double hbm_budget_bytes, erod_server** out);
 int erod_server_destroy(erod_server* s);
 /* The pool batch (for its tile configuration).  The pool steps in the
  * library's environment (gravity, contact), taken anew at every tick. */
@@ -274,6 +308,8 @@ int erod_server_status(const erod_server* s, uint64_t* queued, uint64_t* running
  * (each rod counts min(steps per tick, its budget left) steps times its active
  * length), so that throughput over a window is a difference of two readings. */
 int erod_server_stats(const erod_server* s, uint64_t* ticks, uint64_t* element_steps);
+/* Slots in the pool, pool rebuilds so far (growths), and the HBM budget. */
+int erod_server_pool_info(const erod_server* s, uint64_t* slots, uint64_t* growths, double* hbm_budget_bytes);
 
 /* ---- checkpoints ------------------------------------------------------------ */
 

@@ -594,7 +594,14 @@ namespace oracle {
   // ---------------------------------------------------------------------------
   // Module hooks restated as descriptors (Constraints / Forcing / Environments)
   // ---------------------------------------------------------------------------
-  struct Clamp {  // value + rate constraint at node/element `index` of a rod
+  // value + rate constraint of a fixed clamp at index i of a rod: node i and
+  // element min(i, n - 1) (a clamp at the last node, i = n, holds the last
+  // element with it, as a clamp at i = 0 holds the first).  The constraint's
+  // per-index callbacks
+  // constrain_primitives / constrain_rates
+  // (elastica/Simulator/Constraints/Constraint.hpp:290-313,
+  // elastica/Simulator/Interface/Constraints/ExecutionPolicy.hpp:591-593).
+  struct Clamp {
     std::size_t rod, index;
     double x[3], Q[9];
   };
@@ -639,8 +646,9 @@ namespace oracle {
       auto& b = s.blocks[bi];
       const std::size_t N = b.E + 1, E = b.E;
       const std::size_t st = b.starts[ri];
+      const std::size_t e = std::min(c.index, b.n_elems[ri] - 1);
       for (int d = 0; d < 3; ++d) b.get(Position)[d * N + st + c.index] = T(c.x[d]);
-      for (int m = 0; m < 9; ++m) b.get(Director)[m * E + st + c.index] = T(c.Q[m]);
+      for (int m = 0; m < 9; ++m) b.get(Director)[m * E + st + e] = T(c.Q[m]);
     }
   }
 
@@ -651,8 +659,9 @@ namespace oracle {
       auto& b = s.blocks[bi];
       const std::size_t N = b.E + 1, E = b.E;
       const std::size_t st = b.starts[ri];
+      const std::size_t e = std::min(c.index, b.n_elems[ri] - 1);
       for (int d = 0; d < 3; ++d) b.get(Velocity)[d * N + st + c.index] = T(0);
-      for (int d = 0; d < 3; ++d) b.get(AngularVelocity)[d * E + st + c.index] = T(0);
+      for (int d = 0; d < 3; ++d) b.get(AngularVelocity)[d * E + st + e] = T(0);
     }
   }
 

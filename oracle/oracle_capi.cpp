@@ -216,6 +216,8 @@ int oracle_kinematic_update(void* h, double dt) {
 int oracle_add_clamp(void* h, uint64_t rod, uint64_t index, const double* x3, const double* q9) {
   auto* s = static_cast<Sys*>(h);
   if (rod >= s->where.size()) return -1;
+  auto [bi, ri] = s->where[rod];
+  if (index > s->blocks[bi].n_elems[ri]) return -1;  // past the rod's last node
   oracle::Clamp c{};
   c.rod = rod;
   c.index = index;

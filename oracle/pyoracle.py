@@ -187,7 +187,8 @@ class System:
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(3)
         Q = np.ascontiguousarray(Q, dtype=np.float64).reshape(9)
         if self._L.oracle_add_clamp(self._h, rod, index, _dptr(x), _dptr(Q)) != 0:
-            raise IndexError("clamp on rod %d of a system of %d rods" % (rod, len(self.n_elems)))
+            raise IndexError("clamp at index %d of rod %d of a system of %d rods (index past the rod's last node, "
+                             "or no such rod)" % (index, rod, len(self.n_elems)))
 
     def add_end_force(self, rod, F, ramp=0, start=0.0, interval=1.0):
         F = np.ascontiguousarray(F, dtype=np.float64).reshape(3)

@@ -26,7 +26,7 @@ def rotated(wl, R):
         s["Velocity"] = R @ st["Velocity"]
         s["Director"] = _turn_directors(st["Director"], R)
         states.append(s)
-    clamps = [(r, R @ x, _turn_directors(np.asarray(Q).reshape(9, 1), R)[:, 0]) for r, x, Q in wl.clamps]
+    clamps = [(r, R @ x, _turn_directors(np.asarray(Q).reshape(9, 1), R)[:, 0], *i) for r, x, Q, *i in wl.clamps]
     forces = [(r, R @ np.asarray(F), ramp, t0, T) for r, F, ramp, t0, T in wl.forces]
     gravity = None if wl.gravity is None else tuple(R @ np.asarray(wl.gravity))
     contact = None

@@ -21,7 +21,10 @@ def _pack(n_elems, version=2):
     b += np.array([7] * n_rods, "<u4").tobytes() + b"\x01"
     b += np.arange(n_rods, dtype="<f8").tobytes()
     for r in range(n_rods):
-        b += struct.pack("<3dI3d9d3di2d", 1e-3 * (r + 1), 0.1, 0.2, 2, 0, 0, 0, *([0.0] * 9), 1.0, 2.0, 3.0, 1, 0.0, 5.0)
+        flags = 2 | (1 | 8 if version >= 3 and r == 0 else 0)
+        b += struct.pack("<3dI3d9d3di2d", 1e-3 * (r + 1), 0.1, 0.2, flags, 0, 0, 0, *([0.0] * 9), 1.0, 2.0, 3.0, 1, 0.0, 5.0)
+        if version >= 3:
+            b += struct.pack("<2I3d9d", 0, n_elems[r], 0.5, 0.25, r, *np.eye(3).ravel())
     b += struct.pack("<i3di6d", 1, 0.0, 0.0, -9.81, 0, 0, 0, 0, 0, 0, 0)
     if version >= 2:
         b += struct.pack("<3d", 0.6, 0.0, 0.8)
@@ -50,6 +53,18 @@ def test_reader_parses_the_documented_layout(tmp_path):
         assert np.array_equal(ck["fields"][name], a), name
     x = checkpoint.rod_field(ck, "Position", 1)
     assert np.array_equal(x, fields["Position"][:, 7:11])
+
+
+def test_reader_reads_the_clamp_indices_of_version_three(tmp_path):
+    data, fields = _pack([5, 3], version=3)
+    p = tmp_path / "v3.ckpt"
+    p.write_bytes(data)
+    ck = checkpoint.read(str(p))
+    assert ck["version"] == 3 and ck["rods"][0]["flags"] == 11
+    assert ck["rods"][0]["clamp_index"] == (0, 5) and ck["rods"][1]["clamp_index"] == (0, 3)
+    assert list(ck["rods"][1]["clamp1_x"]) == [0.5, 0.25, 1.0]
+    assert np.array_equal(ck["rods"][0]["clamp1_q"], np.eye(3).ravel())
+    assert np.array_equal(ck["fields"]["Director"], fields["Director"])
 
 
 def test_reader_reads_version_one_with_the_horizontal_plane(tmp_path):

@@ -84,8 +84,9 @@ def test_reader_returns_device_contents(tmp_path):
             assert np.array_equal(checkpoint.rod_field(ck, tag, r), a.get_rod(tag, r)), (r, tag)
     assert ck["env"]["gravity"] and ck["env"]["contact"] and ck["env"]["plane"]["k"] == 1e3
     n = np.array([0.1, 0.0, 1.0]) / np.sqrt(1.01)
-    assert np.allclose(ck["env"]["plane"]["normal"], n, rtol=0, atol=1e-16) and ck["version"] == 2
+    assert np.allclose(ck["env"]["plane"]["normal"], n, rtol=0, atol=1e-16) and ck["version"] == 3
     assert ck["rods"][0]["flags"] & 1 and ck["rods"][1]["ramp"] == 2 and ck["rods"][0]["dt"] == wl.dt
+    assert ck["rods"][0]["clamp_index"] == (0, 0) and not ck["rods"][0]["flags"] & 8
 
 
 def test_load_rejects_a_foreign_file(tmp_path):
@@ -96,23 +97,27 @@ def test_load_rejects_a_foreign_file(tmp_path):
 
 
 def test_version_one_file_loads_with_the_level_plane(tmp_path):
-    """A version 1 checkpoint (written before the plane normal existed: the
-    same layout without the normal's 24 bytes) loads and steps as the batch
-    it came from with its plane set level."""
+    """A version 1 checkpoint (written before the plane normal and the clamp
+    indices existed: the same layout without the 104 bytes of clamp indices
+    and second clamp after each rod record and without the normal's 24
+    bytes) loads and steps as the batch it came from with its plane set
+    level."""
     import struct
     wl = _hooked()
     wl.contact = tuple(wl.contact[:6])  # level plane
     a = build_block(wl)
     a.step_fused(50, FP64)
-    path = tmp_path / "v2.ckpt"
+    path = tmp_path / "v3.ckpt"
     a.save(str(path))
     data = bytearray(path.read_bytes())
     n = len(wl.n_elems)
     # header 32 B; n_elems, n_active (u64), steps_left (u32); budgets (u8);
-    # rod clocks (f64); 168 B per rod record; environment 80 B; then the normal
-    at = 32 + n * (8 + 8 + 4) + 1 + 8 * n + 168 * n + 80
+    # rod clocks (f64); 168 + 104 B per rod record; environment 80 B; then the normal
+    head = 32 + n * (8 + 8 + 4) + 1 + 8 * n
+    recs = b"".join(bytes(data[head + 272 * r:head + 272 * r + 168]) for r in range(n))
+    at = head + 272 * n + 80
     assert struct.unpack_from("<3d", data, at) == (0.0, 0.0, 1.0)
-    v1 = data[:8] + struct.pack("<I", 1) + data[12:at] + data[at + 24:]
+    v1 = data[:8] + struct.pack("<I", 1) + data[12:head] + recs + data[head + 272 * n:at] + data[at + 24:]
     p1 = tmp_path / "v1.ckpt"
     p1.write_bytes(bytes(v1))
     assert checkpoint.read(str(p1))["version"] == 1

@@ -56,7 +56,13 @@ def test_uniform_batch_against_oracle():
     sysm.step(wl.dt, wl.steps, THREADS)
     err = rel_state_error(fused, sysm)
     assert err["Position"] <= 1e-10 and err["Director"] <= 1e-10, err
-    assert err["Velocity"] <= 1e-6 and err["AngularVelocity"] <= 1e-6, err
+    # the C-library oracle: velocities within the spread that two builds of
+    # the oracle differing only in the last bits of sin / cos / acos show on
+    # this batch after 10,000 steps (3.4e-8 and 5.9e-8,
+    # tests/test_oracle_conditioning.py); against the oracle built with the
+    # device library's functions the fused path is bitwise equal
+    # (tests/test_gpu_device_math.py)
+    assert err["Velocity"] <= 1e-7 and err["AngularVelocity"] <= 1e-7, err
     # the op-by-op path against the fused one on the device (same math
     # library, so the conditioning above does not enter)
     ops = build_block(wl)

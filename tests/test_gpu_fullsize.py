@@ -49,13 +49,22 @@ def test_ragged_fused_fp64_equals_op_by_op(ragged, fused64):
     ops.close()
 
 
+def _rod_position_errors(a, b, wl):
+    """per rod: max over its nodes of |x_a - x_b|, over the rod's length"""
+    xa, xb = a.get_field("Position"), b.get_field("Position")
+    err = np.empty(len(wl.n_elems))
+    for r, (s0, n) in enumerate(zip(b.starts, wl.n_elems)):
+        L = wl.states[r]["ReferenceElementLength"].sum()
+        err[r] = np.abs(xa[:, s0:s0 + n + 1] - xb[:, s0:s0 + n + 1]).max() / L
+    return err
+
+
 def test_ragged_fp32_bulk_close_to_fp64(ragged, fused64):
     blk = build_block(ragged)
     blk.step_fused_graph(STEPS, 100, FP32_BULK)
     assert not blk.check_finite().any()
-    x32, x64 = blk.get_field("Position"), fused64.get_field("Position")
-    err = np.abs(x32 - x64).max() / np.abs(x64).max()
-    assert err <= 1e-6, err
+    err = _rod_position_errors(blk, fused64, ragged)
+    assert err.max() <= 1e-6, (int(err.argmax()), err.max())
     blk.close()
 
 
@@ -71,20 +80,19 @@ def test_ragged_rods_alone_equal_in_batch(ragged, fused64):
 
 def test_fp32_bulk_error_after_ten_thousand_steps():
     """The accuracy target of the FP32-bulk path over the full horizon: after
-    10,000 steps of a 256-rod batch from the same ragged generator, each rod
-    with its own stable dt, every rod's positions stay within 1e-6 of its
-    length of the FP64 trajectory."""
-    wl = ragged_4096(n_rods=256, steps=10000)
+    10,000 steps of the whole 4,096-rod ragged batch, each rod at its own
+    stable dt, every rod's positions stay within 1e-6 of its length of the
+    FP64 trajectory."""
+    wl = ragged_4096(steps=10000)
     a, b = build_block(wl), build_block(wl)
     for blk in (a, b):
         blk.set_rod_dt(wl.info["rod_dt"])
     a.step_fused_graph(10000, 100, FP32_BULK)
     b.step_fused_graph(10000, 100, FP64)
-    assert not a.check_finite().any()
-    worst = 0.0
-    for r in range(256):
-        L = wl.states[r]["ReferenceElementLength"].sum()
-        worst = max(worst, np.abs(a.get_rod("Position", r) - b.get_rod("Position", r)).max() / L)
-    assert worst <= 1e-6, worst
+    assert not a.check_finite().any() and not b.check_finite().any()
+    err = _rod_position_errors(a, b, wl)
+    print("fp32-bulk vs fp64 after 10,000 steps, per rod |dx|/L: max %.3e, median %.3e"
+          % (err.max(), np.median(err)), flush=True)
+    assert err.max() <= 1e-6, (int(err.argmax()), err.max())
     a.close()
     b.close()

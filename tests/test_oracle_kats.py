@@ -205,6 +205,44 @@ def test_timoshenko_cantilever_deflection_converges():
     assert 1.8 <= e10 / e20 <= 2.2, (e10, e20)
 
 
+def _clamped_clamped_error(n):
+    """A beam clamped at both ends (clamps at index 0 and index n, the last
+    holding the last node with the last element) under its weight in a weak
+    gravity field, run to rest."""
+    L, r, rho, E, G, gx = 3.0, 0.25, 5e3, 1e6, 1e4, -0.01
+    s = pyoracle.System([n])
+    s.init_straight_rod(0, r, L, [0, 0, 0], [0, 0, 1], [0, 1, 0], rho, E, G, 0.5, 0.5)
+    x0, Q0 = s.rod_field(0, "Position").copy(), s.rod_field(0, "Director").copy()
+    s.add_clamp(0, x0[:, 0], Q0[:, 0], 0)
+    s.add_clamp(0, x0[:, n], Q0[:, n - 1], n)
+    s.set_gravity([gx, 0.0, 0.0])
+    dt = 0.01 * L / n
+    s.step(dt, int(200.0 / dt))
+    x = s.rod_field(0, "Position")
+    A = np.pi * r * r
+    I = A * A / (4 * np.pi)
+    q = rho * A * abs(gx)
+    z = x[2]
+    want = -(q * z ** 2 * (L - z) ** 2 / (24 * E * I) + q * z * (L - z) / (2 * (27.0 / 28.0) * G * A))
+    held = np.array_equal(x[:, 0], x0[:, 0]) and np.array_equal(x[:, n], x0[:, n]) and \
+        np.array_equal(s.rod_field(0, "Director")[:, n - 1], Q0[:, n - 1])
+    return np.abs(x[0] - want).max(), np.abs(want).max(), held
+
+
+def test_beam_clamped_at_both_ends_converges():
+    """Clamps at any index: a uniform load on a beam clamped at both ends,
+    against the shearable closed form w(s) = q s^2 (L-s)^2 / (24 EI) +
+    q s (L-s) / (2 a G A), a = 27/28 (bending as for an Euler-Bernoulli
+    beam clamped at both ends, shear from V(s) = q (L/2 - s)): the error
+    is about 2% at n = 20 and halves when the element count doubles.  A last
+    end held at its node alone (a pinned end) is 21% off."""
+    e10, wmax, held10 = _clamped_clamped_error(10)
+    e20, _, held20 = _clamped_clamped_error(20)
+    assert held10 and held20
+    assert e20 <= 0.025 * wmax, (e20, wmax)
+    assert 1.6 <= e10 / e20 <= 2.4, (e10, e20)
+
+
 # --- environment hooks (gravity, plane contact, friction) ----------------------------
 
 def test_free_fall_is_exact_under_position_verlet():

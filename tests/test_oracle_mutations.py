@@ -42,6 +42,15 @@ ENV_MUTATIONS = [
     ("fe[c * N + i] += T(s.g[c]) * m[i];", "fe[c * N + i] += T(s.g[c]);", "gravity without the mass"),
 ]
 
+# mistakes in the clamps at an index, which the golden vectors (clamp-free)
+# do not reach; the clamped-clamped beam's closed form must catch them
+CLAMP_MUTATIONS = [
+    ("const std::size_t e = std::min(c.index, b.n_elems[ri] - 1);\n      for (int d = 0; d < 3; ++d) b.get(Position)",
+     "const std::size_t e = c.index;\n      for (int d = 0; d < 3; ++d) b.get(Position)", "last end held at its node alone"),
+    ("b.get(Position)[d * N + st + c.index] = T(c.x[d]);", "b.get(Position)[d * N + st + e] = T(c.x[d]);",
+     "the element's index for the node"),
+]
+
 FLAGS = ["-O1", "-ffp-contract=off", "-std=c++20", "-fPIC", "-pthread", "-shared"]
 
 
@@ -90,7 +99,7 @@ def _failing_env_kats():
 @pytest.fixture(scope="module")
 def builds(tmp_path_factory):
     tmp = str(tmp_path_factory.mktemp("oracle_mut"))
-    jobs = [None] + MUTATIONS + ENV_MUTATIONS
+    jobs = [None] + MUTATIONS + ENV_MUTATIONS + CLAMP_MUTATIONS
     with concurrent.futures.ThreadPoolExecutor(max_workers=4) as ex:
         libs = list(ex.map(lambda im: _build(tmp, im[0], im[1]), enumerate(jobs)))
     yield libs
@@ -120,5 +129,31 @@ def test_planted_environment_mistake_is_caught(builds, i):
     pyoracle.use_library(builds[1 + len(MUTATIONS) + i])
     try:
         assert _failing_env_kats() > 0, ENV_MUTATIONS[i][2]
+    finally:
+        pyoracle.use_library(None)
+
+
+def _clamp_kat_fails():
+    from tests import test_oracle_kats as kats
+    try:
+        kats.test_beam_clamped_at_both_ends_converges()
+    except AssertionError:
+        return True
+    return False
+
+
+def test_unchanged_copy_passes_the_clamp_kat(builds):
+    pyoracle.use_library(builds[0])
+    try:
+        assert not _clamp_kat_fails()
+    finally:
+        pyoracle.use_library(None)
+
+
+@pytest.mark.parametrize("i", range(len(CLAMP_MUTATIONS)), ids=[m[2] for m in CLAMP_MUTATIONS])
+def test_planted_clamp_mistake_is_caught(builds, i):
+    pyoracle.use_library(builds[1 + len(MUTATIONS) + len(ENV_MUTATIONS) + i])
+    try:
+        assert _clamp_kat_fails(), CLAMP_MUTATIONS[i][2]
     finally:
         pyoracle.use_library(None)
