@@ -253,6 +253,18 @@ def contact_leg(args, torch):
     log("rods on a plane: %d rods x %d, %.3f s for %d steps -> %.3e element-steps/s, VALU issue %s"
         % (len(n), n[0], el, args.contact_steps, v,
            "%.2f" % roofline["frac"] if roofline["frac"] is not None else "(no counter summary for this shape)"))
+    # FP32-bulk against fused FP64 over the same steps (each rod's positions
+    # relative to its length), before the plain steps below move blk on
+    from elasticapp_amd import FP64
+    b64c = make_block(wl)
+    b64c.set_fused_config(0, 0, args.chunk)
+    b64c.step_fused_graph(args.chunk + args.contact_steps, args.chunk, FP64)
+    x32, x64 = blk.get_field("Position"), b64c.get_field("Position")
+    b64c.close()
+    Lc = np.array([st["ReferenceElementLength"].sum() for st in wl.states])
+    cerr = np.array([np.abs(x32[:, s0:s0 + k + 1] - x64[:, s0:s0 + k + 1]).max() for s0, k in zip(blk.starts, n)]) / Lc
+    log("rods on a plane: fp32-bulk vs fp64 after %d steps: max |dx|/L %.3e (median %.3e)"
+        % (args.chunk + args.contact_steps, cerr.max(), np.median(cerr)))
     # plain FP64 steps continue from the fused state on the same batch
     blk.step(wl.dt, 2)
     sync()
@@ -267,6 +279,8 @@ def contact_leg(args, torch):
             "elements": elems, "dt": wl.dt, "steps": args.contact_steps, "value": v, "unit": "element-steps/s",
             "ms_per_step": el * 1e3 / args.contact_steps, "tile_lanes": lanes, "elements_per_thread": epl,
             "steps_per_launch": spl, "min_z": zmin,
+            "fp32_bulk_position_err_vs_fp64": {"max_over_rods": float(cerr.max()), "median_over_rods": float(np.median(cerr)),
+                                               "steps": args.chunk + args.contact_steps},
             "roofline": roofline,
             "fp64_plain": {"value": vp, "steps": args.contact_plain_steps}, "ratio_fp32bulk_vs_plain_fp64": v / vp}
 
@@ -315,13 +329,21 @@ def small_batches_leg(args, torch, cpu):
     err = rel_state_error(blk, sysm)
     plain = build_block(wl)
     elp = timed(lambda: plain.step(wl.dt, 1000), lambda: (plain.synchronize(), torch.cuda.synchronize()))
+    # FP32-bulk over the same steps: its tip against FP64, relative to the rod length
+    b32c = build_block(wl)
+    b32c.step_fused(wl.steps, FP32_BULK)
+    L0 = float(wl.states[0]["ReferenceElementLength"].sum())
+    err32 = float(np.abs(b32c.get_rod("Position", 0) - blk.get_rod("Position", 0)).max() / L0)
+    b32c.close()
     out["timoshenko_cantilever"] = {"steps": wl.steps, "fp64_fused": 100 * wl.steps / el, "fp64_plain": 100 * 1000 / elp,
                             "unit": "element-steps/s", "max_rel_err_vs_oracle": max(err.values()),
+                            "fp32_bulk_position_err_vs_fp64": err32,
                             "tip": blk.get_rod("Position", 0)[:, -1].tolist(),
                             "cpu_baseline": {"value": 100 * wl.steps / cpu_el, "unit": "element-steps/s", "cores": 1,
                                              "kind": "port", "sample": "the whole run, 1 thread"}}
-    log("timoshenko cantilever (100 elements): fp64 fused %.3e, plain %.3e element-steps/s, max rel err vs oracle %.1e"
-        % (out["timoshenko_cantilever"]["fp64_fused"], out["timoshenko_cantilever"]["fp64_plain"], max(err.values())))
+    log("timoshenko cantilever (100 elements): fp64 fused %.3e, plain %.3e element-steps/s, max rel err vs oracle %.1e, "
+        "fp32-bulk position error %.2e" % (out["timoshenko_cantilever"]["fp64_fused"], out["timoshenko_cantilever"]["fp64_plain"],
+                                            max(err.values()), err32))
     wl = uniform_1024x64(steps=10000)
     n = np.array(wl.n_elems)
     blk = build_block(wl)
@@ -525,6 +547,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=450.0, help="Poisson arrival rate of the serving leg (1/s)")
     ap.add_argument("--small-batches", type=int, default=1, help="1: also run the cantilever and the uniform batch")
     ap.add_argument("--tile", default="", help="LANES,EPL: fixed tile shape instead of the autotuner's choice")
+    ap.add_argument("--tile64", default="", help="LANES,EPL[,SPLIT_LANES,SPLIT_EPL]: fixed fused FP64 plan")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rods", type=int, default=4096)
     args = ap.parse_args()
@@ -592,8 +615,15 @@ def main():
     # ---- fused FP64 over the same steps ----------------------------------------------------
     blk64 = make_block(wl)
     blk64.set_rod_dt(rod_dt)
-    blk64.set_fused_config(0, 0, spl)
-    blk64.autotune(FP64, spl)
+    if args.tile64:
+        # a fixed plan (counter runs time each dispatch apart, which misleads the autotuner)
+        t64 = [int(x) for x in args.tile64.split(",")]
+        blk64.set_fused_config(t64[0], t64[1], spl)
+        if len(t64) == 4:
+            blk64.set_fused_split(t64[2], t64[3])
+    else:
+        blk64.set_fused_config(0, 0, spl)
+        blk64.autotune(FP64, spl)
 
     def sync64():
         blk64.synchronize()

@@ -553,6 +553,9 @@ static int fused_common(erod_batch* b, int precision) {
   if (b->budgets && b->budgets_dirty) {
     EROD_HIP(hipMemcpyAsync(b->d_steps_left, b->steps_left.data(), sizeof(uint32_t) * b->n_rods, hipMemcpyHostToDevice,
                             b->stream));
+    // the host mirror is spent as soon as the launches are queued: the copy
+    // must have read it before
+    EROD_HIP(hipStreamSynchronize(b->stream));
     b->budgets_dirty = false;
   }
   return fused_prepare(b, precision);

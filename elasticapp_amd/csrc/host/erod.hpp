@@ -172,6 +172,13 @@ namespace erod {
       check(erod_server_create(library.handle(), cap.data(), cnt.data(), int(classes.size()), p, steps_per_tick,
                                hbm_budget_bytes, &s_));
     }
+    // a pool of the given slot capacities that grows on demand while it fits
+    // the HBM budget (erod_server_create_budgeted)
+    Server(DeviceBlock& library, const std::vector<std::uint64_t>& capacities, erod_precision p, int steps_per_tick,
+           double hbm_budget_bytes) {
+      check(erod_server_create_budgeted(library.handle(), capacities.data(), int(capacities.size()), p, steps_per_tick,
+                                        hbm_budget_bytes, &s_));
+    }
     Server(const Server&) = delete;
     Server& operator=(const Server&) = delete;
     ~Server() { erod_server_destroy(s_); }
