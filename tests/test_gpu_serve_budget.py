@@ -72,6 +72,32 @@ def test_budgeted_pool_grows_and_results_do_not_change(precision):
         alone.close()
 
 
+def test_pool_grows_while_requests_run():
+    """Requests arrive over several ticks, so that the pool grows while
+    earlier requests are half way through their steps: the rods moved into
+    the new pool continue bit for bit (each request equals the same request
+    served alone)."""
+    wl, lib = _library()
+    rng = np.random.default_rng(59)
+    reqs = [(i, [int(r) for r in rng.integers(0, 24, int(rng.integers(3, 9)))], int(rng.integers(150, 400)))
+            for i in range(36)]
+    srv = Server.budgeted(lib, [32, 64, 128], precision=FP32_BULK, steps_per_tick=50, hbm_budget_bytes=2e9)
+    results, finished, _ = _run(srv, reqs, submit_at={i: i // 6 for i in range(36)})
+    info = srv.pool_info()
+    srv.close()
+    assert info["growths"] >= 2, info
+    for rid in (0, 5, 11, 23, 35):
+        rods, steps = reqs[rid][1], reqs[rid][2]
+        alone = Server(lib, [(32, 16), (64, 16), (128, 16)], precision=FP32_BULK, steps_per_tick=50,
+                       hbm_budget_bytes=2e9)
+        got, _, _ = _run(alone, [(rid, rods, steps)])
+        for k in range(len(rods)):
+            for t in STATE:
+                assert np.isfinite(got[rid][k][t]).all()
+                assert np.array_equal(results[rid][k][t], got[rid][k][t]), (rid, k, t)
+        alone.close()
+
+
 def test_budget_bounds_the_pool():
     """A budget that holds a little more than the initial pool: the pool
     grows once or twice, never past the budget, and the requests beyond it
