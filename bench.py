@@ -209,7 +209,7 @@ def cpu_baseline(wl, budget_s=12.0, n_sample=512):
     t = time.perf_counter()
     sysm.step(wl.dt, 5, threads)
     probe = time.perf_counter() - t
-    steps = int(max(10, min(5000, budget_s / max(probe / 5, 1e-6))))
+    steps = int(max(10, min(50000, budget_s / max(probe / 5, 1e-6))))
     t = time.perf_counter()
     sysm.step(wl.dt, steps, threads)
     el = time.perf_counter() - t

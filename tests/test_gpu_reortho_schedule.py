@@ -74,6 +74,27 @@ def test_graph_schedule_equals_direct(precision, every, chunk):
         blk.close()
 
 
+@pytest.mark.parametrize("precision", [FP64, FP32_BULK])
+def test_schedule_over_rod_pieces(precision):
+    """Rods longer than the tile run as pieces (several launches per chunk,
+    two state copies): the scheduled step equals stepping by the interval and
+    calling erod_reorthonormalize between, bitwise."""
+    wl = random_workload(5, 300, 1500, seed=711, steps=0)
+    a, b = build_block(wl), build_block(wl)
+    for blk in (a, b):
+        blk.set_fused_config(256, 1, 40)
+        blk.set_fused_pieces(256, 1, 16)
+    a.set_reortho_interval(80)
+    a.step_fused_graph(240, 80, precision)
+    for _ in range(3):
+        b.step_fused(80, precision)
+        b.reorthonormalize()
+    assert a.fused_pieces()[4] > 5
+    _equal(a, b, range(5))
+    for blk in (a, b):
+        blk.close()
+
+
 def test_spent_budgets_keep_their_state():
     wl = random_workload(6, 30, 200, seed=705, steps=0)
     blk = build_block(wl)
