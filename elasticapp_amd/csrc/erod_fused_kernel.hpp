@@ -734,6 +734,14 @@ namespace erod {
       }
       if constexpr (!kCache) __syncthreads();  // every lookup in the rod table done before s1 is written
       if (tid < 9) s3[tid][0] = R(0);  // read only after the barriers of the first step
+      // whether a lane of this wave holds a clamped node or element: a
+      // wave-uniform value, so that the waves without clamps (most of a
+      // batch) branch over the clamp updates instead of selecting through
+      // them every step
+      uint32_t cbits = 0;
+#pragma unroll
+      for (int j = 0; j < P; ++j) cbits |= rflags[j];
+      const bool wclamp = __ballot((cbits & (0x3fu << 8)) != 0u) != 0ull;
       for (int step = 0; step < nsteps; ++step) {
         // a rod whose budget is spent keeps its state (its element forces are
         // still evaluated, since the tile moves in lockstep, but never applied)
@@ -756,6 +764,7 @@ namespace erod {
               for (int c = 0; c < 3; ++c) xt[c] = maddd<R>(ch, vt[c], xt[c]);
           }
           tm[j] += 0.5 * dt[j];
+          if (!wclamp) continue;
           if (rflags[j] & (3u << 8)) {
             const RodInfo& ri = rods[rodi[j]];
             int k = int((rflags[j] >> 8) & 3u) - 1;
@@ -823,10 +832,17 @@ namespace erod {
 #pragma unroll
           for (int j = 0; j < P; ++j) geo[kGeo ? j : 0][4] = j + 1 < P ? geo[kGeo ? (j + 1 < P ? j + 1 : j) : 0][3] : s5[kCache ? 3 : 0][tn];
         }
+        // Idle lanes (past the tile's last element) compute on their own
+        // stand-in state and constants too: the valid lanes are a prefix of
+        // the tile, so an idle lane's terms reach only idle lanes (its own
+        // later elements and the next thread's) and are never applied; a
+        // branch that zeroed them would cost every wave its moves each step.
         Mid<R> mid[P];
 #pragma unroll
         for (int j = 0; j < P; ++j) {
-          if (!valid[j]) {
+          if (kEnv == 2 && !valid[j]) {
+            // (the tilted plane's kernel keeps the branch: without it, the
+            // compiler's schedule for it needs more than 256 registers)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
               mid[j].c[c] = mid[j].P[c] = mid[j].Rv[c] = mid[j].shear[c] = mid[j].lag[c] = mid[j].unst[c] = R(0);
@@ -1028,17 +1044,19 @@ namespace erod {
               const R a = (fn + static_cast<R>(fen[c])) * tail_im;
               vt[c] = maddd<R>(dtt, static_cast<double>(a), vt[c]);
             }
-            if (rflags[j] & (3u << 10))
+            if (wclamp && (rflags[j] & (3u << 10)))
 #pragma unroll
               for (int c = 0; c < 3; ++c) vt[c] = 0.0;
           }
           // post-dynamic clamp
-          if (rflags[j] & (3u << 8))
+          if (wclamp) {
+            if (rflags[j] & (3u << 8))
 #pragma unroll
-            for (int c = 0; c < 3; ++c) v[j][c] = 0.0;
-          if (rflags[j] & (3u << 12))
+              for (int c = 0; c < 3; ++c) v[j][c] = 0.0;
+            if (rflags[j] & (3u << 12))
 #pragma unroll
-            for (int c = 0; c < 3; ++c) w[j][c] = 0.0;
+              for (int c = 0; c < 3; ++c) w[j][c] = 0.0;
+          }
         }
         // ---- time, kinematic half step ------------------------------------------
         // FP32-bulk: unless this is the rod's last step of the launch, also take
