@@ -243,6 +243,10 @@ namespace {
     p->plan.seg_lanes = old->plan.seg_lanes;
     p->plan.seg_epl = old->plan.seg_epl;
     p->plan.halo = old->plan.halo;
+    p->plan.tuned_shape = old->plan.tuned_shape;
+    // and so does the re-orthonormalisation schedule, at its phase
+    p->reortho_every = old->reortho_every;
+    p->reortho_phase = old->reortho_phase;
     // (requests admitted in this tick but not yet copied hold slots whose
     // rods arrive after the growth)
     std::vector<uint64_t> used;

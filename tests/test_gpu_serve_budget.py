@@ -98,6 +98,32 @@ def test_pool_grows_while_requests_run():
         alone.close()
 
 
+def test_pool_settings_survive_growth():
+    """The tile shape and the re-orthonormalisation schedule set on the pool
+    carry over to the pool a growth builds."""
+    wl, lib = _library()
+    srv = Server.budgeted(lib, [32, 64, 128], precision=FP32_BULK, steps_per_tick=50, hbm_budget_bytes=2e9)
+    srv.pool.set_fused_config(256, 1, 50)
+    srv.pool.set_reortho_interval(100)
+    rng = np.random.default_rng(61)
+    for i in range(40):
+        srv.submit(i, [int(r) for r in rng.integers(0, 24, int(rng.integers(2, 9)))], int(rng.integers(120, 200)))
+    srv.tick()  # admits what fits, growing the pool, and steps it
+    assert srv.pool_info()["growths"] >= 1
+    assert srv.pool.reortho_interval()[0] == 100
+    assert srv.pool.fused_config() == (256, 1, 50)
+    done = 0
+    for _ in range(200):
+        for rid in srv.tick():
+            assert np.isfinite(srv.read(rid, 0, "Position")).all()
+            srv.release(rid)
+            done += 1
+        if done == 40:
+            break
+    assert done == 40
+    srv.close()
+
+
 def test_budget_bounds_the_pool():
     """A budget that holds a little more than the initial pool: the pool
     grows once or twice, never past the budget, and the requests beyond it
