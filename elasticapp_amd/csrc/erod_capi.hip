@@ -218,6 +218,9 @@ int erod_batch_create(const uint64_t* n_elems, uint64_t n_rods, int damped, erod
   if ((rc = hip_check(hipMalloc(&b->d_flags, sizeof(int) * n_rods), "hipMalloc flags"))) return fail(rc);
   if ((rc = hip_check(hipEventCreate(&b->ev0), "hipEventCreate"))) return fail(rc);
   if ((rc = hip_check(hipEventCreate(&b->ev1), "hipEventCreate"))) return fail(rc);
+  if ((rc = hip_check(hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking), "hipStreamCreate"))) return fail(rc);
+  if ((rc = hip_check(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming), "hipEventCreate"))) return fail(rc);
+  if ((rc = hip_check(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming), "hipEventCreate"))) return fail(rc);
   if ((rc = erod_fill_ghosts(b))) return fail(rc);
   if ((rc = hip_check(hipStreamSynchronize(b->stream), "sync"))) return fail(rc);
   *out = b;
@@ -227,6 +230,7 @@ int erod_batch_create(const uint64_t* n_elems, uint64_t n_rods, int damped, erod
 int erod_batch_destroy(erod_batch* b) {
   if (!b) return EROD_OK;
   if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->stream2) hipStreamSynchronize(b->stream2);
   if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
   for (int f = 0; f < EROD_N_FIELDS; ++f) hipFree(b->d_field[f]);
   hipFree(b->d_ghost_nodes);
@@ -250,6 +254,9 @@ int erod_batch_destroy(erod_batch* b) {
   for (double* p : b->d_shadow) hipFree(p);
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);
+  if (b->ev_fork) hipEventDestroy(b->ev_fork);
+  if (b->ev_join) hipEventDestroy(b->ev_join);
+  if (b->stream2) hipStreamDestroy(b->stream2);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
   return EROD_OK;

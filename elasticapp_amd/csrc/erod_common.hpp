@@ -192,6 +192,11 @@ struct erod_batch {
   double* d_shadow[4] = {};
   // timing of the dominant kernel
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // a second stream, and the events that fork a plan's tile groups onto it and
+  // join them back: the groups hold disjoint rods, so the whole-rod tiles run
+  // beside the chain of launches over rod pieces
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   uint64_t last_launches = 0;
   // scratch
   int* d_flags = nullptr;

@@ -1163,17 +1163,17 @@ namespace erod {
 
     template <typename R, int P, int NT, int kEnv>
     inline void launch_env(erod_batch* b, const Consts<R>& C, int steps, const TileGroup& g, const StatePtrs& in,
-                           const StatePtrs& out) {
+                           const StatePtrs& out, hipStream_t st) {
       uint32_t* budgets = b->budgets ? b->d_steps_left : nullptr;
       if (g.d_segs) {
         // pieces: clocks to the second array, budgets spent afterwards
-        hipLaunchKernelGGL((k_fused<R, P, NT, kEnv>), dim3(static_cast<unsigned>(g.n_tiles)), dim3(NT), 0, b->stream, in,
+        hipLaunchKernelGGL((k_fused<R, P, NT, kEnv>), dim3(static_cast<unsigned>(g.n_tiles)), dim3(NT), 0, st, in,
                            out, C, b->d_rods, g.d_tiles, g.d_tile_rods, g.d_tile_lane0, g.d_segs, b->d_time, b->d_time_next,
                            budgets, nullptr, b->env, steps, b->E);
-        hipLaunchKernelGGL(k_seg_clocks, dim3(static_cast<unsigned>((g.n_seg_rods + 255) / 256)), dim3(256), 0, b->stream,
+        hipLaunchKernelGGL(k_seg_clocks, dim3(static_cast<unsigned>((g.n_seg_rods + 255) / 256)), dim3(256), 0, st,
                            g.d_seg_rods, g.n_seg_rods, b->d_time_next, b->d_time, budgets, uint32_t(steps));
       } else {
-        hipLaunchKernelGGL((k_fused<R, P, NT, kEnv>), dim3(static_cast<unsigned>(g.n_tiles)), dim3(NT), 0, b->stream, in,
+        hipLaunchKernelGGL((k_fused<R, P, NT, kEnv>), dim3(static_cast<unsigned>(g.n_tiles)), dim3(NT), 0, st, in,
                            out, C, b->d_rods, g.d_tiles, g.d_tile_rods, g.d_tile_lane0, nullptr, b->d_time, b->d_time,
                            budgets, budgets, b->env, steps, b->E);
       }
@@ -1184,13 +1184,13 @@ namespace erod {
     // plane runs one without the arithmetic of a general normal
     template <typename R, int P, int NT>
     inline int launch_tiles(erod_batch* b, const Consts<R>& C, int steps, const TileGroup& g, const StatePtrs& in,
-                            const StatePtrs& out) {
+                            const StatePtrs& out, hipStream_t st) {
       if (g.n_tiles == 0) return EROD_OK;
       const Env& e = b->env;
       const bool level = e.n[0] == 0.0 && e.n[1] == 0.0 && e.n[2] == 1.0;
-      if (e.contact && !level) launch_env<R, P, NT, 2>(b, C, steps, g, in, out);
-      else if (e.gravity || e.contact) launch_env<R, P, NT, 1>(b, C, steps, g, in, out);
-      else launch_env<R, P, NT, 0>(b, C, steps, g, in, out);
+      if (e.contact && !level) launch_env<R, P, NT, 2>(b, C, steps, g, in, out, st);
+      else if (e.gravity || e.contact) launch_env<R, P, NT, 1>(b, C, steps, g, in, out, st);
+      else launch_env<R, P, NT, 0>(b, C, steps, g, in, out, st);
       return hip_check(hipGetLastError(), "k_fused");
     }
 
@@ -1203,16 +1203,16 @@ namespace erod {
 
     template <typename R>
     inline int launch_shape(erod_batch* b, const Consts<R>& C, int steps, int shape, const TileGroup& g,
-                            const StatePtrs& in, const StatePtrs& out) {
+                            const StatePtrs& in, const StatePtrs& out, hipStream_t st) {
       switch (shape) {
-        case 0: return launch_tiles<R, 1, 64>(b, C, steps, g, in, out);
-        case 1: return launch_tiles<R, 1, 128>(b, C, steps, g, in, out);
-        case 2: return launch_tiles<R, 1, 256>(b, C, steps, g, in, out);
-        case 3: return launch_tiles<R, 1, 512>(b, C, steps, g, in, out);
-        case 4: return launch_tiles<R, 2, 256>(b, C, steps, g, in, out);
-        case 5: return launch_tiles<R, 2, 512>(b, C, steps, g, in, out);
-        case 6: return launch_tiles<R, 4, 128>(b, C, steps, g, in, out);
-        case 7: return launch_tiles<R, 4, 256>(b, C, steps, g, in, out);
+        case 0: return launch_tiles<R, 1, 64>(b, C, steps, g, in, out, st);
+        case 1: return launch_tiles<R, 1, 128>(b, C, steps, g, in, out, st);
+        case 2: return launch_tiles<R, 1, 256>(b, C, steps, g, in, out, st);
+        case 3: return launch_tiles<R, 1, 512>(b, C, steps, g, in, out, st);
+        case 4: return launch_tiles<R, 2, 256>(b, C, steps, g, in, out, st);
+        case 5: return launch_tiles<R, 2, 512>(b, C, steps, g, in, out, st);
+        case 6: return launch_tiles<R, 4, 128>(b, C, steps, g, in, out, st);
+        case 7: return launch_tiles<R, 4, 256>(b, C, steps, g, in, out, st);
         default: set_error("bad tile shape"); return EROD_ERR_INVALID;
       }
     }
@@ -1221,10 +1221,10 @@ namespace erod {
     // with a limit on its steps per launch (pieces of rods), launches of at
     // most that many
     template <typename R>
-    inline int launch_group(erod_batch* b, const Consts<R>& C, int steps, const TileGroup& g) {
+    inline int launch_group(erod_batch* b, const Consts<R>& C, int steps, const TileGroup& g, hipStream_t st) {
       if (g.shape < 0 || g.n_tiles == 0) return EROD_OK;
       const StatePtrs fields = field_state(b);
-      if (!g.d_segs) return launch_shape(b, C, steps, g.shape, g, fields, fields);
+      if (!g.d_segs) return launch_shape(b, C, steps, g.shape, g, fields, fields, st);
       // pieces: launches of at most max_steps steps, each reading the state
       // the previous one wrote and writing the other copy; the state ends in
       // the fields
@@ -1233,12 +1233,13 @@ namespace erod {
       bool in_fields = true;
       for (int done = 0; done < steps;) {
         const int s = steps - done < cap ? steps - done : cap;
-        if (int rc = launch_shape(b, C, s, g.shape, g, in_fields ? fields : shadow, in_fields ? shadow : fields)) return rc;
+        if (int rc = launch_shape(b, C, s, g.shape, g, in_fields ? fields : shadow, in_fields ? shadow : fields, st))
+          return rc;
         in_fields = !in_fields;
         done += s;
       }
       if (!in_fields) {
-        hipLaunchKernelGGL(k_copy_cut_state, dim3(static_cast<unsigned>(g.n_seg_rods)), dim3(256), 0, b->stream, g.d_seg_rods,
+        hipLaunchKernelGGL(k_copy_cut_state, dim3(static_cast<unsigned>(g.n_seg_rods)), dim3(256), 0, st, g.d_seg_rods,
                            b->d_rods, shadow, fields, b->E);
         return hip_check(hipGetLastError(), "k_copy_cut_state");
       }
@@ -1246,13 +1247,32 @@ namespace erod {
     }
 
     // `steps` steps over every tile of the plan: its main group, its second
-    // group, the pieces of its long rods (rods never interact, so the order
-    // does not matter)
+    // group, the pieces of its long rods.  Rods never interact, so the groups
+    // run side by side: with pieces, the whole-rod groups go to the batch's
+    // second stream beside the chain of piece launches; without, the second
+    // group goes there beside the main one.  Both streams join back before
+    // the call returns (in a captured graph: two branches and a join).
     template <typename R>
     inline int launch_all(erod_batch* b, const Consts<R>& C, int steps) {
-      for (const TileGroup& g : b->plan.g)
-        if (int rc = launch_group(b, C, steps, g)) return rc;
-      return EROD_OK;
+      const TileGroup* g = b->plan.g;
+      auto busy = [](const TileGroup& t) { return t.shape >= 0 && t.n_tiles > 0; };
+      const bool pieces = busy(g[2]);
+      const int side = (pieces ? busy(g[0]) : 0) + busy(g[1]);
+      if (side == 0 || (!pieces && !busy(g[0]))) {
+        for (int k = 0; k < 3; ++k)
+          if (int rc = launch_group(b, C, steps, g[k], b->stream)) return rc;
+        return EROD_OK;
+      }
+      EROD_HIP(hipEventRecord(b->ev_fork, b->stream));
+      EROD_HIP(hipStreamWaitEvent(b->stream2, b->ev_fork, 0));
+      int rc = EROD_OK;
+      if (pieces) rc = launch_group(b, C, steps, g[0], b->stream2);
+      if (rc == EROD_OK) rc = launch_group(b, C, steps, g[1], b->stream2);
+      int rc2 = launch_group(b, C, steps, pieces ? g[2] : g[0], b->stream);
+      // the join is recorded whatever failed, so that a capture ends whole
+      EROD_HIP(hipEventRecord(b->ev_join, b->stream2));
+      EROD_HIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
+      return rc ? rc : rc2;
     }
 
   }  // namespace fused
