@@ -373,12 +373,23 @@ namespace erod {
     uint64_t cap2 = s2 >= 0 ? uint64_t(fused_shape_lanes(s2)) : 0;
     if (cap2 >= cap) cap2 = 0;
     const uint64_t min2 = cap2 ? uint64_t(fused_shape_epl(s2)) : 0;
+    // each entry takes a whole number of its group's threads: its length
+    // rounded up to the elements per thread (the kernel puts the idle pad
+    // lanes before the entry's elements, so that its last element is a
+    // thread's last)
+    auto padded = [](uint64_t n, uint64_t p) { return static_cast<uint32_t>((n + p - 1) / p * p); };
+    std::vector<uint32_t> len1(len.size());
+    for (const uint32_t i : whole) len1[i] = padded(len[i], uint64_t(fused_shape_epl(si)));
     Packed g1, g2, g3;
-    pack(len, cap, longest_first(len, whole, cap), cap2, min2, g1);
-    if (!g1.overflow.empty()) pack(len, cap2, g1.overflow, 0, 0, g2);
+    pack(len1, cap, longest_first(len1, whole, cap), cap2, min2, g1);
+    if (!g1.overflow.empty()) {
+      std::vector<uint32_t> len2(len.size());
+      for (const uint32_t i : g1.overflow) len2[i] = padded(len[i], uint64_t(fused_shape_epl(s2)));
+      pack(len2, cap2, g1.overflow, 0, 0, g2);
+    }
     std::vector<uint32_t> slen(segs.size()), sids(segs.size());
     for (uint32_t k = 0; k < segs.size(); ++k) {
-      slen[k] = segs[k].n;
+      slen[k] = padded(segs[k].n, uint64_t(fused_shape_epl(ps)));
       sids[k] = k;
     }
     if (!segs.empty()) pack(slen, pcap, longest_first(slen, sids, pcap), 0, 0, g3);

@@ -564,7 +564,7 @@ namespace erod {
       // FP32-bulk: reciprocals of the element constants, formed once per launch
       R inv_m[kCache ? P : 1], inv_l0[kCache ? P : 1], inv_D0[kCache ? P : 1], inv_J[kCache ? P : 1][3];
       double xt[3] = {0, 0, 0}, vt[3] = {0, 0, 0};
-      int tailj = -1;
+      int tailj = -1;  // P - 1 when the thread holds its rod's last element, else -1
       // FP32-bulk without environment: the end force on the rod's last node
       // is kept in registers for the launch (the ramp's times are read only
       // while a ramp is set), instead of two dependent rod-record loads per
@@ -593,11 +593,17 @@ namespace erod {
         }
         return lo;
       };
-      // pass 1: which rod and slot each of the thread's elements is
+      // pass 1: which rod and slot each of the thread's elements is.  Every
+      // entry of a tile takes a whole number of threads: its first lane is a
+      // thread's first, and (P - n mod P) mod P idle pad lanes come before its
+      // elements, so that its last element is a thread's last (lane P - 1).
+      // A pad lane computes on stand-in state (zero node state, the entry's
+      // first director) with the entry's first-element constants and no
+      // Voronoi region (zero couple), like an idle lane.
+      bool pad[P];
 #pragma unroll
       for (int j = 0; j < P; ++j) {
         const uint32_t el = uint32_t(tid) * P + j;
-        valid[j] = el < tile.lanes;
         const uint32_t e = find(el);
         const uint32_t ent = t_rod[e];
         uint32_t r, off, segn, w0, w1, ends;
@@ -619,7 +625,10 @@ namespace erod {
         }
         rodi[j] = int(r);
         const RodInfo& ri = rods[r];
-        kk[j] = valid[j] ? int(el - t_lane0[e]) : 0;
+        const uint32_t padn = (uint32_t(P) - segn % uint32_t(P)) % uint32_t(P), raw = el - t_lane0[e];
+        pad[j] = el < tile.lanes && raw < padn;
+        valid[j] = el < tile.lanes && raw >= padn;
+        kk[j] = valid[j] ? int(raw - padn) : 0;
         nn[j] = int(segn);
         slot[j] = ri.start + uint64_t(off) + uint64_t(kk[j]);
         dt[j] = ri.dt;
@@ -638,7 +647,7 @@ namespace erod {
         }
         own[j] = valid[j] && uint32_t(kk[j]) >= w0 && uint32_t(kk[j]) < w1;
         clockw[j] = valid[j] && kk[j] == 0 && (segs == nullptr || (ends & kSegLast) != 0);
-        if (valid[j] && kk[j] == int(segn) - 1) tailw = (ends & kSegLast) != 0;
+        if (j == P - 1 && valid[j] && kk[j] == int(segn) - 1) tailw = (ends & kSegLast) != 0;
         // damping rates: one per rod, or (flag bit 2) per node and element
         const bool per_node = (ri.flags & 4u) != 0 && valid[j];
         gfj[j] = per_node ? C.gf[slot[j]] : static_cast<R>(ri.gamma_f);
@@ -657,10 +666,13 @@ namespace erod {
 #pragma unroll
         for (int j = 0; j < P; ++j) v[j][c] = valid[j] ? in.v[c * N + slot[j]] : 0.0;
       }
+      // (a pad lane takes its entry's first director: the relative rotation
+      // between them is zero, so the pad's log map stays on its series branch)
 #pragma unroll
       for (int m = 0; m < 9; ++m) {
 #pragma unroll
-        for (int j = 0; j < P; ++j) q[j][m] = valid[j] ? in.q[m * E + slot[j]] : ((m % 4 == 0) ? 1.0 : 0.0);
+        for (int j = 0; j < P; ++j)
+          q[j][m] = valid[j] || pad[j] ? in.q[m * E + slot[j]] : ((m % 4 == 0) ? 1.0 : 0.0);
       }
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -669,7 +681,7 @@ namespace erod {
       }
 #pragma unroll
       for (int j = 0; j < P; ++j) {
-        if (valid[j] && kk[j] == nn[j] - 1) {
+        if (j == P - 1 && valid[j] && kk[j] == nn[j] - 1) {
           tailj = j;
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
@@ -693,8 +705,8 @@ namespace erod {
         const uint64_t V = E - 1;
 #pragma unroll
         for (int j = 0; j < P; ++j) {
-          if (!valid[j]) continue;
-          const uint64_t sl = slot[j];
+          if (!valid[j] && !pad[j]) continue;
+          const uint64_t sl = slot[j];  // (a pad lane: the entry's first element)
           const uint32_t ce = uint32_t(j) * NT + tid;  // lane tid P + j
           sc[0][ce] = C.m[sl];
           sc[1][ce] = C.l0[sl];
@@ -704,7 +716,7 @@ namespace erod {
             sc[5 + c][ce] = C.S[c * E + sl];
             sc[8 + c][ce] = C.J[c * E + sl];
           }
-          const bool vor = kk[j] + 1 < nn[j];
+          const bool vor = valid[j] && kk[j] + 1 < nn[j];
           sc[11][ce] = vor ? C.D0[sl] : 1.0f;
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
@@ -721,7 +733,7 @@ namespace erod {
           const uint32_t el = uint32_t(j) * NT + tid;                   // lane tid P + j
           const uint32_t eln = j + 1 < P ? el + NT : uint32_t(tid) + 1;  // lane tid P + j + 1
           const R mk = sc[0][el];
-          const R mkn = j == tailj ? tail_m : sc[0][eln];
+          const R mkn = j == P - 1 && j == tailj ? tail_m : sc[0][eln];
           inv_m[j] = rcp_(mk);
           inv_l0[j] = rcp_(sc[1][el]);
           inv_D0[j] = rcp_(sc[11][el]);
@@ -759,7 +771,7 @@ namespace erod {
 #pragma unroll
             for (int c = 0; c < 3; ++c) x[j][c] = maddd<R>(ch, v[j][c], x[j][c]);
             rodrigues<R>(q[j], w[j], ch);
-            if (j == tailj)
+            if (j == P - 1 && j == tailj)
 #pragma unroll
               for (int c = 0; c < 3; ++c) xt[c] = maddd<R>(ch, vt[c], xt[c]);
           }
@@ -796,7 +808,7 @@ namespace erod {
         constexpr bool kGeo = kCache && P > 1;
         float geo[kGeo ? P : 1][5];
         {
-          const bool last0 = valid[0] && (kk[0] == nn[0] - 1);
+          const bool last0 = P == 1 && valid[0] && (kk[0] == nn[0] - 1);
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
             s1[c][tid] = x[0][c];
@@ -821,7 +833,7 @@ namespace erod {
         if constexpr (kGeo) {
 #pragma unroll
           for (int j = 1; j < P; ++j) {
-            const bool lastj = kk[j] == nn[j] - 1;
+            const bool lastj = j == P - 1 && kk[j] == nn[j] - 1;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
               const double xn = lastj ? xt[c] : (j + 1 < P ? x[j + 1 < P ? j + 1 : j][c] : s1[c][tn]);
@@ -849,7 +861,7 @@ namespace erod {
             mid[j].e = R(1);
             continue;
           }
-          const bool last = kk[j] == nn[j] - 1;
+          const bool last = j == P - 1 && kk[j] == nn[j] - 1;
           double xk1[3], xk2[3] = {0, 0, 0}, vk1[3], qk1[9];
           float dn[3] = {0.0f, 0.0f, 0.0f};
           if (last) {
@@ -871,7 +883,7 @@ namespace erod {
               vk1[c] = s1[3 + c][tn];
             }
           }
-          const bool has_vor = !last;
+          const bool has_vor = !last && !pad[j];
           // the next lane's director and node difference; for a rod's last
           // element they belong to the next rod (or to an idle lane: identity
           // director, zero state) and only meet its zero Voronoi constants
@@ -884,7 +896,7 @@ namespace erod {
 #pragma unroll
               for (int m = 0; m < 9; ++m) qk1[m] = s1[6 + m][tn];
             }
-            const bool next_last = kk[j] + 1 == nn[j] - 1;
+            const bool next_last = (j + 1) % P == P - 1 && kk[j] + 1 == nn[j] - 1;
             if (kCache && j == P - 1 && (P > 1 || next_last)) {
               // FP32-bulk: the next element's rounded difference, from its thread
 #pragma unroll
@@ -951,7 +963,7 @@ namespace erod {
         for (int j = 0; j < P; ++j) {
           if (!live[j]) continue;
           const int k = kk[j];
-          const bool first = k == 0, last = k == nn[j] - 1;
+          const bool first = k == 0, last = j == P - 1 && k == nn[j] - 1;
           // the -1 element's terms (the previous thread's from exchange 2 for
           // j = 0).  Before a rod's first element sits the previous rod's last
           // element, whose P and R are zero (no Voronoi region), or the zero
@@ -1070,7 +1082,7 @@ namespace erod {
 #pragma unroll
           for (int c = 0; c < 3; ++c) x[j][c] = maddd<R>(ch, v[j][c], x[j][c]);
           rodrigues<R>(q[j], w[j], ch);
-          if (j == tailj)
+          if (j == P - 1 && j == tailj)
 #pragma unroll
             for (int c = 0; c < 3; ++c) xt[c] = maddd<R>(ch, vt[c], xt[c]);
         }
@@ -1105,7 +1117,7 @@ namespace erod {
       for (int j = 0; j < P; ++j) {
         if (!valid[j]) continue;
         const uint64_t s = slot[j];
-        if (j == tailj && tailw) {
+        if (j == P - 1 && j == tailj && tailw) {
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
             out.x[c * N + s + 1] = xt[c];
