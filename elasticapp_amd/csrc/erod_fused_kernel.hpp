@@ -1,9 +1,12 @@
 // Fused, persistent position-Verlet step kernel (device code, instantiated
 // for FP64 in erod_fused64.hip and for FP32-bulk in erod_fused32.hip).
 //
-// One workgroup owns a tile of whole rods.  Each thread holds P contiguous
-// elements of the tile (element k, node k and, for the last element of a rod,
-// the rod's last node n) with the state x, v, Q, w in FP64 registers across
+// One workgroup owns a tile of whole rods (or of pieces of long rods), each
+// starting on a thread's first lane and ending on a thread's last (idle pad
+// lanes before an entry whose length is not a multiple of P).  Each thread
+// holds P contiguous lanes of the tile (element k, node k and, for the last
+// element of a rod, the rod's last node n) with the state x, v, Q, w in FP64
+// registers across
 // `nsteps` steps.  The only cross-thread data of a step are the +1 neighbour
 // node values and director and the next element's node difference (first
 // exchange) and the -1 neighbour element force / couple terms (second
@@ -40,9 +43,10 @@
 //   element lengths come
 //   from node differences rounded to FP32, which the neighbour that needs the
 //   same length receives instead of recomputing it.  The FP32 constants are
-//   held in an LDS table for the whole launch (1/m, 1/J, 1/l0, 1/D0 formed
-//   when used), and inside a launch the closing and opening half steps of
-//   consecutive steps are one kinematic update of dt.
+//   held in an LDS table for the whole launch (1/m, 1/J, 1/l0, 1/D0 and the
+//   damping products formed once per launch into registers), and inside a
+//   launch the closing and opening half steps of consecutive steps are one
+//   kinematic update of dt.
 #pragma once
 
 #include "erod_common.hpp"
@@ -532,7 +536,7 @@ namespace erod {
       __shared__ R s3[9][NT + 1];
       // FP32-bulk: 18 constants of every lane of the tile, for the whole launch:
       //   0 m, 1 l0, 2-4 sig0, 5-7 S, 8-10 J, 11 D0, 12-14 kap0, 15-17 B
-      // (1/m, 1/J, 1/l0, 1/D0 are formed when used; the defaults of an element
+      // (1/m, 1/J, 1/l0, 1/D0 are formed from it once per launch; the defaults of an element
       // without a Voronoi region are stored for it).  Lane el = t P + j is kept
       // at j NT + t, so that a wave reading element j of its threads reads
       // consecutive words.
