@@ -758,12 +758,24 @@ namespace erod {
 #pragma unroll
       for (int j = 0; j < P; ++j) cbits |= rflags[j];
       const bool wclamp = __ballot((cbits & (0x3fu << 8)) != 0u) != 0ull;
+      // the write-back predicates as bits of one register (not lane masks the
+      // step loop would carry in scalar registers): bit j element j owned,
+      // bit 8 + j element j writes its rod's clock, bit 16 the tail node;
+      // an element that is no rod's gets no steps
+      uint32_t wb = tailw ? 1u << 16 : 0u;
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        wb |= (own[j] ? 1u : 0u) << j;
+        wb |= (clockw[j] ? 1u : 0u) << (8 + j);
+        lim[j] = valid[j] ? lim[j] : 0u;
+      }
+      asm volatile("" : "+v"(wb));
       for (int step = 0; step < nsteps; ++step) {
         // a rod whose budget is spent keeps its state (its element forces are
         // still evaluated, since the tile moves in lockstep, but never applied)
         bool live[P];
 #pragma unroll
-        for (int j = 0; j < P; ++j) live[j] = valid[j] && uint32_t(step) < lim[j];
+        for (int j = 0; j < P; ++j) live[j] = uint32_t(step) < lim[j];
         // ---- kinematic half step, time, post-kinematic clamp --------------------
         // (FP32-bulk: after the first step of a launch this half step was taken
         // together with the closing half step of the previous one)
@@ -1097,38 +1109,37 @@ namespace erod {
       for (int c = 0; c < 3; ++c) {
 #pragma unroll
         for (int j = 0; j < P; ++j)
-          if (own[j]) out.x[c * N + slot[j]] = x[j][c];
+          if ((wb >> j) & 1u) out.x[c * N + slot[j]] = x[j][c];
       }
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
 #pragma unroll
         for (int j = 0; j < P; ++j)
-          if (own[j]) out.v[c * N + slot[j]] = v[j][c];
+          if ((wb >> j) & 1u) out.v[c * N + slot[j]] = v[j][c];
       }
 #pragma unroll
       for (int m = 0; m < 9; ++m) {
 #pragma unroll
         for (int j = 0; j < P; ++j)
-          if (own[j]) out.q[m * E + slot[j]] = q[j][m];
+          if ((wb >> j) & 1u) out.q[m * E + slot[j]] = q[j][m];
       }
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
 #pragma unroll
         for (int j = 0; j < P; ++j)
-          if (own[j]) out.w[c * E + slot[j]] = w[j][c];
+          if ((wb >> j) & 1u) out.w[c * E + slot[j]] = w[j][c];
       }
 #pragma unroll
       for (int j = 0; j < P; ++j) {
-        if (!valid[j]) continue;
         const uint64_t s = slot[j];
-        if (j == P - 1 && j == tailj && tailw) {
+        if (j == P - 1 && j == tailj && ((wb >> 16) & 1u)) {
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
             out.x[c * N + s + 1] = xt[c];
             out.v[c * N + s + 1] = vt[c];
           }
         }
-        if (clockw[j]) {
+        if ((wb >> (8 + j)) & 1u) {
           time_out[rodi[j]] = tm[j];
           if (steps_out && lim[j] != kNoBudget) steps_out[rodi[j]] = lim[j] > uint32_t(nsteps) ? lim[j] - uint32_t(nsteps) : 0u;
         }
