@@ -860,11 +860,13 @@ namespace erod {
 #pragma unroll
           for (int j = 0; j < P; ++j) geo[kGeo ? j : 0][4] = j + 1 < P ? geo[kGeo ? (j + 1 < P ? j + 1 : j) : 0][3] : s5[kCache ? 3 : 0][tn];
         }
-        // Idle lanes (past the tile's last element) compute on their own
-        // stand-in state and constants too: the valid lanes are a prefix of
-        // the tile, so an idle lane's terms reach only idle lanes (its own
-        // later elements and the next thread's) and are never applied; a
-        // branch that zeroed them would cost every wave its moves each step.
+        // Idle lanes (past the tile's last entry) and pad lanes compute on
+        // their stand-in state and constants too.  A tile's entries end on a
+        // thread's last lane, so an idle lane's terms reach only idle lanes;
+        // a pad lane's reach only its entry's first element, which masks the
+        // force term and reads couple terms that are zero (the pad has no
+        // Voronoi region).  None are applied; a branch that zeroed them would
+        // cost every wave its moves each step.
         Mid<R> mid[P];
 #pragma unroll
         for (int j = 0; j < P; ++j) {
