@@ -8,9 +8,12 @@
  * (Block.hpp:401-443); each field is component-major (component c of slot k
  * at c * slots + k; a director is 9 lines, line 3p + r holding Q(p, r)).
  *
- * All calls return EROD_OK (0) or a negative erod_status.  A batch owns one
+ * All calls return EROD_OK (0) or a negative erod_status.  A batch owns a
  * HIP stream; calls on one batch are ordered on that stream and are not
- * thread-safe.  Host pointers are plain host memory.
+ * thread-safe.  (A fused launch whose plan has several tile groups forks
+ * some of them onto a second stream of the batch and joins it back before
+ * the call returns, so the order holds.)  Host pointers are plain host
+ * memory.
  */
 #ifndef EROD_H
 #define EROD_H
