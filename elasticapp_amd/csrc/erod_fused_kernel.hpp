@@ -496,6 +496,47 @@ namespace erod {
       }
     }
 
+    // FP32-bulk: the same law, with the node's height above the plane formed
+    // in FP64 (a resting node's gap is a small difference of heights) and
+    // the rest in FP32 like the path's other loads: no FP64 root or division
+    template <bool kLevel>
+    __device__ __forceinline__ void node_environment32(const Env& env, float m, const double x[3], const double v[3],
+                                                       float fe[3]) {
+      if (env.gravity)
+        for (int c = 0; c < 3; ++c) fe[c] = madd(static_cast<float>(env.g[c]), m, fe[c]);
+      if (!env.contact) return;
+      double xn;
+      if constexpr (kLevel) xn = x[2];
+      else xn = (env.n[0] * x[0] + env.n[1] * x[1]) + env.n[2] * x[2];
+      const double gap = xn - env.z0 - env.offset;
+      if (gap < 0.0) {
+        const float delta = static_cast<float>(-gap);
+        const float vv[3] = {static_cast<float>(v[0]), static_cast<float>(v[1]), static_cast<float>(v[2])};
+        const float nn[3] = {static_cast<float>(env.n[0]), static_cast<float>(env.n[1]), static_cast<float>(env.n[2])};
+        const float vnv = kLevel ? vv[2] : madd(nn[2], vv[2], madd(nn[1], vv[1], nn[0] * vv[0]));
+        const float fn = fmaxf(0.0f, madd(static_cast<float>(env.k), delta, static_cast<float>(env.c) * -vnv));
+        float vt[3], vtn;
+        if constexpr (kLevel) {
+          vt[0] = vv[0];
+          vt[1] = vv[1];
+          vt[2] = 0.0f;
+          vtn = sqrt_(madd(vt[1], vt[1], vt[0] * vt[0]));
+          fe[2] += fn;
+        } else {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) vt[c] = madd(-vnv, nn[c], vv[c]);
+          vtn = sqrt_(madd(vt[2], vt[2], madd(vt[1], vt[1], vt[0] * vt[0])));
+#pragma unroll
+          for (int c = 0; c < 3; ++c) fe[c] = madd(fn, nn[c], fe[c]);
+        }
+        if (vtn > 0.0f) {
+          const float s = fminf(static_cast<float>(env.ct) * vtn, static_cast<float>(env.mu) * fn) * rcp_(vtn);
+#pragma unroll
+          for (int c = 0; c < (kLevel ? 2 : 3); ++c) fe[c] = madd(-s, vt[c], fe[c]);
+        }
+      }
+    }
+
     // ---- the kernel ------------------------------------------------------------------
 
     // kEnv: 0 no environment, 1 gravity and / or contact with a level plane,
@@ -1033,14 +1074,24 @@ namespace erod {
             tau[c] = madd(-eg, static_cast<R>(w[j][c]), ((dq + md.shear[c]) + md.lag[c]) + md.unst[c]);
           }
           // external loads of node k (forcing, then environment)
-          double fe[3] = {0.0, 0.0, 0.0};
-          if constexpr (kEnv != 0) node_environment<kEnv == 1>(env, static_cast<double>(mk), x[j], v[j], fe);
           const double dtt = dt[j];
+          if constexpr (kCache) {
+            float fe[3] = {0.0f, 0.0f, 0.0f};
+            if constexpr (kEnv != 0) node_environment32<kEnv == 1>(env, mk, x[j], v[j], fe);
 #pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            // (without an environment the external load is zero)
-            const R a = (kEnv != 0 ? f[c] + static_cast<R>(fe[c]) : f[c]) * im;
-            v[j][c] = maddd<R>(dtt, static_cast<double>(a), v[j][c]);
+            for (int c = 0; c < 3; ++c) {
+              // (without an environment the external load is zero)
+              const R a = (kEnv != 0 ? f[c] + fe[c] : f[c]) * im;
+              v[j][c] = maddd<R>(dtt, static_cast<double>(a), v[j][c]);
+            }
+          } else {
+            double fe[3] = {0.0, 0.0, 0.0};
+            if constexpr (kEnv != 0) node_environment<kEnv == 1>(env, static_cast<double>(mk), x[j], v[j], fe);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              const R a = (kEnv != 0 ? f[c] + static_cast<R>(fe[c]) : f[c]) * im;
+              v[j][c] = maddd<R>(dtt, static_cast<double>(a), v[j][c]);
+            }
           }
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
@@ -1067,12 +1118,23 @@ namespace erod {
                 for (int c = 0; c < 3; ++c) fen[c] = ri.force[c] * rv;
               }
             }
-            if constexpr (kEnv != 0) node_environment<kEnv == 1>(env, static_cast<double>(tail_m), xt, vt, fen);
+            if constexpr (kCache) {
+              float fenv[3] = {0.0f, 0.0f, 0.0f};
+              if constexpr (kEnv != 0) node_environment32<kEnv == 1>(env, tail_m, xt, vt, fenv);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-              const R fn = madd(-mgn, static_cast<R>(vt[c]), -md.c[c]);
-              const R a = (fn + static_cast<R>(fen[c])) * tail_im;
-              vt[c] = maddd<R>(dtt, static_cast<double>(a), vt[c]);
+              for (int c = 0; c < 3; ++c) {
+                const R fn = madd(-mgn, static_cast<R>(vt[c]), -md.c[c]);
+                const R a = (kEnv != 0 ? (fn + static_cast<R>(fen[c])) + fenv[c] : fn + static_cast<R>(fen[c])) * tail_im;
+                vt[c] = maddd<R>(dtt, static_cast<double>(a), vt[c]);
+              }
+            } else {
+              if constexpr (kEnv != 0) node_environment<kEnv == 1>(env, static_cast<double>(tail_m), xt, vt, fen);
+#pragma unroll
+              for (int c = 0; c < 3; ++c) {
+                const R fn = madd(-mgn, static_cast<R>(vt[c]), -md.c[c]);
+                const R a = (fn + static_cast<R>(fen[c])) * tail_im;
+                vt[c] = maddd<R>(dtt, static_cast<double>(a), vt[c]);
+              }
             }
             if (wclamp && (rflags[j] & (3u << 10)))
 #pragma unroll
