@@ -14,7 +14,7 @@ def test_more_than_one_gpu_is_refused():
     assert "one GPU" in r.stderr and r.stdout == ""
 
 
-def test_driver_step_counts_pass_the_argument_checks():
+def test_twenty_steps_after_five_warmup_pass_the_argument_checks():
     """`--steps 20 --warmup 5` (fewer steps than a chunk, not whole chunks)
     gets past the argument checks: without a GPU the run stops later, at the
     device, not with the usage exit code 2 or an argument message."""
