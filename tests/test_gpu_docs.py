@@ -9,7 +9,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-GUIDE = "INTEGRATION" + ".md"
+GUIDE = "INTEGRATION.md"  # the file the examples are read from
 
 
 def _python_blocks():
