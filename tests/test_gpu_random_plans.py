@@ -3,7 +3,8 @@
 Each case draws a batch whose rod lengths include the tile-shape edges
 (2, 3, P*NT - 1, P*NT, P*NT + 1 and rods cut into several pieces), random
 hooks (clamps at any index, one or two per rod, end forces with and without
-ramps, damping rates along some rods, gravity, sometimes a contact plane) and
+ramps, damping rates along some rods, gravity, sometimes a contact plane,
+level or of any normal) and
 a random fused plan (main tile shape, second group, piece shape and halo,
 steps per launch).  Then:
 
@@ -67,9 +68,15 @@ def _case(seed):
             st["TorqueDampingRate"] = rng.uniform(0.0, 0.8, (1, n))
     if rng.random() < 0.5:
         wl.gravity = tuple(rng.normal(0.0, 0.5, 3))
-    if rng.random() < 0.3:
-        lowest = min(float(st["Position"][2].min()) for st in wl.states)
-        wl.contact = (1e3, 10.0, 0.3, 10.0, lowest + 1e-5, 0.0)  # the lowest nodes just inside
+    if rng.random() < 0.4:
+        # a level plane, or one of any normal (the kernels' general law); the
+        # lowest nodes just inside it
+        nrm = np.array([0.0, 0.0, 1.0])
+        if rng.random() < 0.5:
+            nrm = rng.normal(size=3)
+            nrm /= np.linalg.norm(nrm)
+        lowest = min(float((nrm @ st["Position"]).min()) for st in wl.states)
+        wl.contact = (1e3, 10.0, 0.3, 10.0, lowest + 1e-5, 0.0) + ((tuple(nrm),) if nrm[2] != 1.0 else ())
     # the plan: a second group of smaller tiles, pieces of a shape and halo
     # that fit, steps per launch
     smaller = [s for s in SHAPES if s[0] < cap and s[1] <= p]
