@@ -654,8 +654,9 @@ def main():
                      "fp32_bulk": elems * rest / els32, "fp64_fused": elems * rest / els64,
                      "fp32_bulk_kernel_ms_per_launch": kms_s32, "fp32_bulk_launches": launches_s32,
                      "unit": "element-steps/s",
-                     "note": "the steps from the timed ones up to the error horizon, in hipGraph chunks of "
-                             "steps_per_launch steps"}
+                     "note": "the steps from the timed ones up to the error horizon, in erod_step_fused_graph "
+                             "chunks of steps_per_launch steps (a chunk of one kernel launch goes out as "
+                             "that launch, a chunk of several as a hipGraph replay)"}
         log("sustained over %d more steps at %d steps/launch: fp32-bulk %.3e, fp64 fused %.3e element-steps/s"
             % (rest, args.chunk, sustained["fp32_bulk"], sustained["fp64_fused"]))
     flags = blk32.check_finite()

@@ -179,6 +179,7 @@ struct erod_batch {
   // graph cache
   hipGraphExec_t graph_exec = nullptr;
   uint64_t graph_launches = 0;  // kernel launches in one replay of graph_exec
+  bool graph_one_node = false;  // graph_exec is a single kernel node
   int graph_precision = -1;
   uint64_t graph_chunk = 0;
   uint64_t graph_reortho[2] = {0, 0};  // interval and phase the graph was captured at

@@ -214,7 +214,9 @@ int erod_autotune_fused(erod_batch* b, int precision, int trial_steps, double* b
 int erod_step_fused(erod_batch* b, int precision, uint64_t n);
 /* Same, replaying a captured hipGraph of `chunk` steps n / chunk times, then
  * launching the n % chunk steps left directly (in launches of
- * steps_per_launch, the last one shorter). */
+ * steps_per_launch, the last one shorter).  A chunk whose graph is a single
+ * kernel (one launch, one tile group) goes out as that kernel's launch
+ * instead of a replay: same work, less device time. */
 int erod_step_fused_graph(erod_batch* b, int precision, uint64_t n, uint64_t chunk);
 /* Capture the hipGraph of one chunk of `chunk` fused steps now (as
  * erod_step_fused_graph would on its first replay), without launching it, so

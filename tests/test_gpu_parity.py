@@ -263,14 +263,18 @@ def test_fp32_bulk_launch_segmentation_is_a_rounding_effect():
         assert np.abs(x - y).max() <= 1e-9 * max(1.0, np.abs(y).max()), r
 
 
-def test_fused_graph_replay_equals_direct():
+@pytest.mark.parametrize("spl", [10, 20])
+def test_fused_graph_replay_equals_direct(spl):
+    """Chunks of 20 steps: two launches replayed as a captured graph (10 steps
+    per launch), or one launch, which goes out as that launch (20)."""
     wl = random_workload(20, 2, 120, seed=13, steps=60)
     a = device_block(wl)
-    a.set_fused_config(0, 0, 20)
+    a.set_fused_config(0, 0, spl)
     a.step_fused(60, FP32_BULK)
     b = device_block(wl)
-    b.set_fused_config(0, 0, 20)
+    b.set_fused_config(0, 0, spl)
     b.step_fused_graph(60, 20, FP32_BULK)
+    assert b.last_kernel_ms()[1] == 60 // spl
     for r in range(len(wl.n_elems)):
         for t in STATE:
             assert np.array_equal(a.get_rod(t, r), b.get_rod(t, r)), (r, t)
@@ -300,14 +304,14 @@ def test_fused_graph_takes_steps_that_do_not_fill_a_chunk(precision):
 
 @pytest.mark.parametrize("precision", [FP32_BULK, FP64])
 def test_graph_replays_with_eager_work_between(precision):
-    """One captured chunk replayed four times, each replay followed by eager
-    work on the device (another batch's fused steps, the finiteness check with
-    its clear, downloads): every replay equals the eager path advanced by the
-    same steps."""
+    """One captured chunk (two launches of 10 steps) replayed four times, each
+    replay followed by eager work on the device (another batch's fused steps,
+    the finiteness check with its clear, downloads): every replay equals the
+    eager path advanced by the same steps."""
     wl = random_workload(20, 2, 120, seed=19, steps=0)
     a, b = device_block(wl), device_block(wl)
     for blk in (a, b):
-        blk.set_fused_config(0, 0, 20)
+        blk.set_fused_config(0, 0, 10)
     other = device_block(random_workload(5, 4, 30, seed=20, steps=0))
     for rep in range(4):
         a.step_fused(20, precision)
