@@ -495,7 +495,7 @@ def test_split_plan_equals_single_plan(precision):
         blk.close()
 
 
-def test_autotune_keeps_a_second_group_only_if_faster():
+def test_autotuned_plan_steps_like_the_default_and_keeps_a_set_second_group():
     """On a batch of short rods no second group can help (every rod fits the
     smallest tiles' room), and the autotuner's plan steps bitwise like the
     default plan; a second group set by the caller is kept as set."""
