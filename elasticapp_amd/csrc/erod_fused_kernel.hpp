@@ -86,6 +86,16 @@ namespace erod {
 
     __device__ __forceinline__ float rcp_(float x) { return __builtin_amdgcn_rcpf(x); }
     __device__ __forceinline__ double rcp_(double x) { return 1.0 / x; }
+
+    // one word per lane from global memory straight into LDS, without a
+    // register (global_load_lds_dword): lane l of the wave writes wave_dst[l],
+    // wave_dst being the same for every lane of the wave; lanes that are off
+    // write nothing
+    __device__ __forceinline__ void load_to_lds(const float* src, float* wave_dst) {
+      // (C-style casts: they convert between address spaces)
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                       (__attribute__((address_space(3))) void*)wave_dst, 4, 0, 0);
+    }
     __device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
     __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
 
@@ -724,6 +734,46 @@ namespace erod {
 #pragma unroll
         for (int j = 0; j < P; ++j) w[j][c] = valid[j] ? in.w[c * E + slot[j]] : 0.0;
       }
+      // FP32-bulk: the thread's constants into the LDS table, from the slots
+      // found above, loaded straight into LDS right after the state loads:
+      // nothing waits between them, so both are in flight together, and the
+      // table takes no registers.  An element without a Voronoi region gets
+      // the defaults written instead (other entries: the two writes never meet)
+      if constexpr (kCache) {
+        const uint64_t V = E - 1;
+        const uint32_t wave0 = uint32_t(tid) & ~63u;
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          const uint64_t sl = slot[j];  // (a pad lane: the entry's first element)
+          const uint32_t cw = uint32_t(j) * NT + wave0;  // lane (wave0 + l) P + j is entry cw + l
+          if (valid[j] || pad[j]) {
+            load_to_lds(C.m + sl, &sc[0][cw]);
+            load_to_lds(C.l0 + sl, &sc[1][cw]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              load_to_lds(C.sig0 + c * E + sl, &sc[2 + c][cw]);
+              load_to_lds(C.S + c * E + sl, &sc[5 + c][cw]);
+              load_to_lds(C.J + c * E + sl, &sc[8 + c][cw]);
+            }
+          }
+          if (valid[j] && kk[j] + 1 < nn[j]) {
+            load_to_lds(C.D0 + sl, &sc[11][cw]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              load_to_lds(C.kap0 + c * V + sl, &sc[12 + c][cw]);
+              load_to_lds(C.B + c * V + sl, &sc[15 + c][cw]);
+            }
+          } else {
+            const uint32_t ce = uint32_t(j) * NT + tid;  // lane tid P + j
+            sc[11][ce] = 1.0f;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              sc[12 + c][ce] = 0.0f;
+              sc[15 + c][ce] = 0.0f;
+            }
+          }
+        }
+      }
 #pragma unroll
       for (int j = 0; j < P; ++j) {
         if (j == P - 1 && valid[j] && kk[j] == nn[j] - 1) {
@@ -744,31 +794,7 @@ namespace erod {
           }
         }
       }
-      // FP32-bulk: the thread's constants into the LDS table, from the slots
-      // found above
       if constexpr (kCache) {
-        const uint64_t V = E - 1;
-#pragma unroll
-        for (int j = 0; j < P; ++j) {
-          if (!valid[j] && !pad[j]) continue;
-          const uint64_t sl = slot[j];  // (a pad lane: the entry's first element)
-          const uint32_t ce = uint32_t(j) * NT + tid;  // lane tid P + j
-          sc[0][ce] = C.m[sl];
-          sc[1][ce] = C.l0[sl];
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            sc[2 + c][ce] = C.sig0[c * E + sl];
-            sc[5 + c][ce] = C.S[c * E + sl];
-            sc[8 + c][ce] = C.J[c * E + sl];
-          }
-          const bool vor = valid[j] && kk[j] + 1 < nn[j];
-          sc[11][ce] = vor ? C.D0[sl] : 1.0f;
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            sc[12 + c][ce] = vor ? C.kap0[c * V + sl] : 0.0f;
-            sc[15 + c][ce] = vor ? C.B[c * V + sl] : 0.0f;
-          }
-        }
         __syncthreads();  // the constant table is written
         // per element, for the launch: 1/m, 1/J, 1/l0, 1/D0 and the damping
         // products m gf and (m + m_next)/2 gt (the same FP32 operations as
