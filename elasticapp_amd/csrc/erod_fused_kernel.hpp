@@ -656,57 +656,73 @@ namespace erod {
       // first director) with the entry's first-element constants and no
       // Voronoi region (zero couple), like an idle lane.
       bool pad[P];
+      // the fields of the rod records first, for the P elements together, so
+      // that their loads go out before any of them is used (one round trip,
+      // not a chain of loads each consumed under a branch)
+      uint32_t e_[P], r_n[P], r_fl[P], r_ci[P][2], s_off[P], s_n[P], s_w0[P], s_w1[P], s_ends[P];
+      uint64_t r_start[P];
+      double r_gf[P], r_gt[P];
 #pragma unroll
       for (int j = 0; j < P; ++j) {
-        const uint32_t el = uint32_t(tid) * P + j;
-        const uint32_t e = find(el);
-        const uint32_t ent = t_rod[e];
-        uint32_t r, off, segn, w0, w1, ends;
+        e_[j] = find(uint32_t(tid) * P + j);
+        const uint32_t ent = t_rod[e_[j]];
+        uint32_t r;
         if (segs) {
           const Seg sg = segs[ent];
           r = sg.rod;
-          off = sg.off;
-          segn = sg.n;
-          w0 = sg.w0;
-          w1 = sg.w1;
-          ends = sg.ends;
+          s_off[j] = sg.off;
+          s_n[j] = sg.n;
+          s_w0[j] = sg.w0;
+          s_w1[j] = sg.w1;
+          s_ends[j] = sg.ends;
         } else {
           r = ent;
-          off = 0;
-          segn = rods[r].n;
-          w0 = 0;
-          w1 = segn;
-          ends = kSegFirst | kSegLast;
+          s_off[j] = 0;
+          s_w0[j] = 0;
+          s_ends[j] = kSegFirst | kSegLast;
         }
         rodi[j] = int(r);
-        const RodInfo& ri = rods[r];
+        const RodInfo* rp = rods + r;
+        r_start[j] = rp->start;
+        r_n[j] = rp->n;
+        r_fl[j] = rp->flags;
+        dt[j] = rp->dt;
+        r_gf[j] = rp->gamma_f;
+        r_gt[j] = rp->gamma_t;
+        r_ci[j][0] = rp->clamp_idx[0];
+        r_ci[j][1] = rp->clamp_idx[1];
+        tm[j] = time_in[r];
+        lim[j] = steps_in ? steps_in[r] : kNoBudget;
+      }
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        const uint32_t el = uint32_t(tid) * P + j, e = e_[j];
+        const uint32_t segn = segs ? s_n[j] : r_n[j], off = s_off[j], ends = s_ends[j];
+        const uint32_t w0 = s_w0[j], w1 = segs ? s_w1[j] : segn;
         const uint32_t padn = (uint32_t(P) - segn % uint32_t(P)) % uint32_t(P), raw = el - t_lane0[e];
         pad[j] = el < tile.lanes && raw < padn;
         valid[j] = el < tile.lanes && raw >= padn;
         kk[j] = valid[j] ? int(raw - padn) : 0;
         nn[j] = int(segn);
-        slot[j] = ri.start + uint64_t(off) + uint64_t(kk[j]);
-        dt[j] = ri.dt;
-        tm[j] = time_in[r];
-        lim[j] = steps_in ? steps_in[r] : kNoBudget;
+        slot[j] = r_start[j] + uint64_t(off) + uint64_t(kk[j]);
         // the end force acts at the rod's last node
-        rflags[j] = ri.flags & ~kClampBits & ((ends & kSegLast) ? ~0u : ~2u);
+        rflags[j] = r_fl[j] & ~kClampBits & ((ends & kSegLast) ? ~0u : ~2u);
         const uint32_t g = off + uint32_t(kk[j]);
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
-          if (!valid[j] || !(ri.flags & clamp_bit(k))) continue;
-          const uint32_t i = ri.clamp_idx[k];
+          if (!valid[j] || !(r_fl[j] & clamp_bit(k))) continue;
+          const uint32_t i = r_ci[j][k];
           if (i == g) rflags[j] |= uint32_t(k + 1) << 8;
           if (kk[j] == int(segn) - 1 && i == off + segn) rflags[j] |= uint32_t(k + 1) << 10;
-          if ((i < ri.n ? i : ri.n - 1) == g) rflags[j] |= uint32_t(k + 1) << 12;
+          if ((i < r_n[j] ? i : r_n[j] - 1) == g) rflags[j] |= uint32_t(k + 1) << 12;
         }
         own[j] = valid[j] && uint32_t(kk[j]) >= w0 && uint32_t(kk[j]) < w1;
         clockw[j] = valid[j] && kk[j] == 0 && (segs == nullptr || (ends & kSegLast) != 0);
         if (j == P - 1 && valid[j] && kk[j] == int(segn) - 1) tailw = (ends & kSegLast) != 0;
         // damping rates: one per rod, or (flag bit 2) per node and element
-        const bool per_node = (ri.flags & 4u) != 0 && valid[j];
-        gfj[j] = per_node ? C.gf[slot[j]] : static_cast<R>(ri.gamma_f);
-        gtj[j] = per_node ? C.gt[slot[j]] : static_cast<R>(ri.gamma_t);
+        const bool per_node = (r_fl[j] & 4u) != 0 && valid[j];
+        gfj[j] = per_node ? C.gf[slot[j]] : static_cast<R>(r_gf[j]);
+        gtj[j] = per_node ? C.gt[slot[j]] : static_cast<R>(r_gt[j]);
       }
       // pass 2: the state, one field line at a time over the P elements (their
       // slots are contiguous within a rod, so the P loads of a line share cache
