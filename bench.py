@@ -569,6 +569,9 @@ def main():
 
     torch.cuda.init()
     dev_name = torch.cuda.get_device_name(0)
+    # compute units this process sees: the persistent launches hold one
+    # workgroup per CU, so their time goes by rounds of tiles over the CUs
+    dev_cus = int(torch.cuda.get_device_properties(0).multi_processor_count)
     log("device", dev_name)
     wl = ragged_4096(n_rods=args.rods, steps=args.steps)
     n = np.array(wl.n_elems)
@@ -744,7 +747,7 @@ def main():
                                   "max": float(rod_dt.max())},
                    "steps_per_launch": spl, "graph_chunk": spl,
                    "tile_lanes": lanes, "elements_per_thread": epl, "second_group": list(split32),
-                   "device": dev_name},
+                   "device": dev_name, "compute_units": dev_cus},
         "roofline": roofline,
         "reortho": reortho,
         "roofline_streaming": {"bound": "hbm", "achieved": achieved_s, "peak": HBM_PEAK_GBS, "unit": "GB/s",
