@@ -43,7 +43,8 @@
 //   element lengths come
 //   from node differences rounded to FP32, which the neighbour that needs the
 //   same length receives instead of recomputing it.  The FP32 constants are
-//   held in an LDS table for the whole launch (1/m, 1/J, 1/l0, 1/D0 and the
+//   held in an LDS table for the whole launch, loaded straight from global
+//   memory into LDS beside the state loads (1/m, 1/J, 1/l0, 1/D0 and the
 //   damping products formed once per launch into registers), and inside a
 //   launch the closing and opening half steps of consecutive steps are one
 //   kinematic update of dt.
